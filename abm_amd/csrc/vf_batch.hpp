@@ -1,0 +1,72 @@
+// vf_batch.hpp -- device-side replica records and the batch object behind the
+// opaque abm_vf_batch handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+
+#include "../../include/abm_amd.h"
+
+namespace abm {
+
+// One replica as the kernels read it (built and validated on the host by
+// abm_vf_batch_create from an abm_vf_replica).
+struct VFRep {
+    int32_t agent_offset;   // first agent in the SoA state
+    int32_t n_agents;       // N
+    int32_t n_rays;         // R
+    int32_t words;          // ceil(R / 32)
+    int32_t table_offset;   // first ray of this R in the ray tables
+    int32_t boundary;       // ABM_BOUNDARY_*
+    int32_t fov_px0, fov_px1;
+    int32_t phi_ok;         // len(arange(-pi, pi, 2pi/R)) == R
+    int32_t limit_movement;
+    int64_t field_offset;   // first uint32 word of agent 0's field
+    double width, height, pad;
+    double gam, v0, alp0, alp1, bet0, bet1;
+    double max_vel, max_th;
+    double step;            // linspace(-pi, pi, R) step
+};
+
+// float32 ray-table entry used by the fused kernel, one per ray.
+struct RayTab32 {
+    float wcos, wsin;   // trapz weight * cos(PHI_k), * sin(PHI_k)
+    float cos, sin;     // cos(PHI_k), sin(PHI_k)
+};
+
+}  // namespace abm
+
+// The opaque handle.
+struct abm_vf_batch {
+    int32_t n_replicas = 0;
+    int32_t n_agents_total = 0;
+    int64_t field_words = 0;
+    int32_t max_agents = 0;
+    int32_t max_rays = 0;
+    int32_t n_tables = 0;
+    int32_t ray_total = 0;          // entries in each ray table
+    int32_t waves_per_group = 0;    // fused kernel launch shape (0 = default)
+    int device = 0;
+    abm::VFRep* reps_dev = nullptr;
+    double* tab_cos = nullptr;      // float64 tables: cos(PHI), sin(PHI), diff(PHI)
+    double* tab_sin = nullptr;
+    double* tab_dphi = nullptr;
+    abm::RayTab32* tab32 = nullptr;
+    unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks
+    int32_t* order_dev = nullptr;   // replica launch order (largest work first)
+    // host copies
+    abm::VFRep* reps_host = nullptr;
+    int32_t* agent_offset_host = nullptr;
+    int64_t* field_offset_host = nullptr;
+};
+
+namespace abm {
+// launchers (vf_f64.hip, vf_f32.hip)
+int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bits,
+                       double* diag, uint32_t flags, hipStream_t s);
+int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
+                       float* diag, uint32_t flags, hipStream_t s);
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+}  // namespace abm

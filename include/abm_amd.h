@@ -1,0 +1,255 @@
+/*
+ * abm_amd.h -- C ABI of the MI355X (gfx950) perception/motion step for
+ * scioip34/ABM agents.
+ *
+ * Plain C: device pointers, sizes and POD descriptors only.  Every entry
+ * point returns 0 on success or a negative ABM_ERR_* code; the message of the
+ * last failure on the calling thread is returned by abm_last_error().  No
+ * entry point throws across the ABI.  Kernels are launched on the caller's
+ * stream (a hipStream_t passed as void*; NULL = the null stream) and are
+ * stream ordered; a call allocates nothing except abm_*_create.
+ *
+ * Each step entry point replaces one call of the reference's per-step agent
+ * update (pygame Group.update over all agents):
+ *   abm_vf_step_f64 / abm_vf_step_f32
+ *       -> VFSimulation.step_sim: self.agents.update(self.agents)
+ *          (abm/projects/visual_flocking/vf_simulation/vf_sims.py:302), i.e.
+ *          VFAgent.update for every agent in id order
+ *          (abm/projects/visual_flocking/vf_agent/vf_agent.py:52-77)
+ *   abm_forage_step_f64
+ *       -> Simulation.start: self.agents.update(self.agents)
+ *          (abm/simulation/sims.py:861), i.e. Agent.update in id order
+ *          (abm/agent/agent.py:212-283)
+ * The probe entry points replace single functions:
+ *   abm_vf_projection_rows   -> vf_supcalc.projection_field
+ *                               (abm/projects/visual_flocking/vf_agent/vf_supcalc.py:20-138)
+ *   abm_vf_state_variables   -> vf_supcalc.VSWRM_flocking_state_variables and
+ *                               dPhi_V_of (vf_supcalc.py:161-277)
+ *   abm_forage_projection    -> Agent.projection_field (abm/agent/agent.py:457-597)
+ */
+#ifndef ABM_AMD_H
+#define ABM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define ABM_OK               0
+#define ABM_ERR_INVALID     -1  /* bad argument, size or descriptor          */
+#define ABM_ERR_HIP         -2  /* a HIP runtime call failed                  */
+#define ABM_ERR_UNSUPPORTED -3  /* valid request this build cannot run        */
+
+/* ------------------------------------------------------------------ enums */
+#define ABM_BOUNDARY_WALLS    0   /* Agent.reflect_from_walls (agent.py:347-395)         */
+#define ABM_BOUNDARY_INFINITE 1   /* torus images + VFAgent.teleport_infinite_arena      */
+
+/* step flags */
+#define ABM_ORDER_SEQUENTIAL  0u  /* reference order: agent k sees agents <k moved      */
+#define ABM_ORDER_SYNCHRONOUS 1u  /* every field from the pre-step poses (not reference) */
+#define ABM_WANT_DIAG         2u  /* write per-agent (dv, dpsi, a_blob, a_edge, b_blob, b_edge) */
+
+/* ----------------------------------------------------- replica description */
+/*
+ * One independent simulation (a "replica") of the visual-flocking model.
+ * Mirrors the VFSimulation / VFAgent / vf_params attributes the step reads:
+ *   n_agents      N                          (Simulation.__init__ N)
+ *   n_rays        R = v_field_res after the 1/FOV scaling (vf_sims.py:40-44)
+ *   fov0, fov1    (-f*pi, f*pi)              (sims.py:161)
+ *   width, height arena size W, H            (agent.py:146-150)
+ *   window_pad    arena offset (30 in the apps, app.py:19)
+ *   boundary      ABM_BOUNDARY_*             (vf_params.BOUNDARY)
+ *   gam .. max_th vf_params constants        (vf_params.py:12-23); ALP2 and BET2
+ *                 multiply an identically zero field (vf_supcalc.py:199) and
+ *                 are therefore not inputs.
+ */
+typedef struct abm_vf_replica {
+    int32_t n_agents;
+    int32_t n_rays;
+    int32_t boundary;
+    int32_t limit_movement;
+    double  fov0, fov1;
+    double  width, height, window_pad;
+    double  gam, v0, alp0, alp1, bet0, bet1;
+    double  max_vel, max_th;
+} abm_vf_replica;
+
+/* Offsets the library assigns when a batch is created (replicas packed in order). */
+typedef struct abm_batch_info {
+    int32_t n_replicas;
+    int32_t n_agents_total;   /* length of every per-agent state array            */
+    int64_t field_words;      /* uint32 words of one packed field set (all agents) */
+    int32_t max_agents;
+    int32_t max_rays;
+} abm_batch_info;
+
+typedef struct abm_vf_batch abm_vf_batch;   /* opaque */
+
+/* Per-agent state, structure of arrays over all agents of all replicas
+ * (replica b owns agents [agent_offset(b), agent_offset(b)+N_b)).  Positions
+ * are sprite top-left corners as in the reference (centre = position + radius).
+ * alp0/bet0/v0 are optional per-agent overrides (VFAgent.ALP0/BET0/V0,
+ * vf_agent.py:24-26); NULL means the replica value. */
+typedef struct abm_vf_state_f64 {
+    double* x; double* y; double* theta; double* v;
+    const int32_t* radius;
+    const double* alp0; const double* bet0; const double* v0;
+} abm_vf_state_f64;
+
+typedef struct abm_vf_state_f32 {
+    float* x; float* y; float* theta; float* v;
+    const int32_t* radius;
+    const float* alp0; const float* bet0; const float* v0;
+} abm_vf_state_f32;
+
+/* Validate host replica descriptors, upload them with the ray tables.
+ * Allocates device memory (the only allocating calls are *_create). */
+int abm_vf_batch_create(const abm_vf_replica* replicas_host, int32_t n_replicas,
+                        abm_vf_batch** out);
+int abm_vf_batch_destroy(abm_vf_batch* batch);
+int abm_vf_batch_get_info(const abm_vf_batch* batch, abm_batch_info* out);
+/* Host copies of the per-replica agent offsets [n_replicas+1] and field word
+ * offsets [n_replicas+1] (agent i of replica b has its field at
+ * field_offset[b] + i*ceil(R_b/32)). */
+int abm_vf_batch_get_offsets(const abm_vf_batch* batch, int32_t* agent_offset,
+                             int64_t* field_offset);
+
+/* One step of every replica, plain float64: the reference arithmetic on the
+ * GPU, one workgroup per replica, per-ray fields in LDS.
+ * field_bits (optional, device, info.field_words uint32): each agent's
+ *   binary field in raw raster order (bit k = ray k, before the reference's
+ *   flip), as used by the integrals; soc_v_field = flip(bits).
+ * diag (optional, device, 6*n_agents_total): with ABM_WANT_DIAG. */
+int abm_vf_step_f64(abm_vf_batch* batch, const abm_vf_state_f64* state,
+                    uint32_t* field_bits, double* diag, uint32_t flags, void* stream);
+
+/* One step of every replica, fused float32 path: bit-packed fields, float32
+ * integrals with compensated sums, float32 poses.  Ray-bin and half-width
+ * decisions are made in float32 with a guard band and re-made with the exact
+ * float64 arithmetic whenever the float32 value falls inside it, so the
+ * rasterised field equals the float64 one on the same poses. */
+int abm_vf_step_f32(abm_vf_batch* batch, const abm_vf_state_f32* state,
+                    uint32_t* field_bits, float* diag, uint32_t flags, void* stream);
+
+/* Count of pair decisions that took the exact float64 re-check in the last
+ * abm_vf_step_f32 on this batch (reads a device counter; synchronises). */
+int abm_vf_f32_fallback_count(abm_vf_batch* batch, int64_t* pairs_total,
+                              int64_t* pairs_rechecked, void* stream);
+
+/* Select the fused kernel's launch shape: waves per workgroup for each size
+ * class (0 = autotune result / default). */
+int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
+/* Time every candidate launch shape on the given state (the state is
+ * advanced by the trial steps) and keep the fastest. */
+int abm_vf_autotune(abm_vf_batch* batch, const abm_vf_state_f32* state, int32_t trial_steps,
+                    int32_t* chosen_waves, float* chosen_ms, void* stream);
+
+/* -------------------------------------------------------------- probes */
+/* Per-object projection rows of one focal agent (vf_supcalc.projection_field).
+ * All pointers are device pointers.  focal = {x, y, theta, radius};
+ * objects: ox[n], oy[n], osize[n].  rows: n_obj * ceil(R/32) uint32, raw
+ * raster order.  rec (optional): n_obj * 5 int32 (phi_target, half, start,
+ * end, drawn) with drawn = -1 for a skipped (coincident) object. */
+int abm_vf_projection_rows(int32_t n_rays, double fov0, double fov1, int32_t boundary,
+                           double width, double height,
+                           double fx, double fy, double ftheta, double fradius,
+                           const double* ox, const double* oy, const double* osize,
+                           int32_t n_obj, uint32_t* rows, int32_t* rec, void* stream);
+
+/* VSWRM_flocking_state_variables for n independent (velocity, field) pairs:
+ * bits: n * ceil(R/32) uint32 raw raster order (V_now); vel[n]; out: n*6
+ * doubles (dvel, dpsi, a_blob, a_edge, b_blob, b_edge).  alp0/bet0/v0 per
+ * row.  dphi (optional): n*R int8 values of dPhi_V_of. */
+int abm_vf_state_variables(int32_t n_rays, int32_t n, const uint32_t* bits, const double* vel,
+                           const double* alp0, const double* bet0, const double* v0,
+                           double gam, double alp1, double bet1,
+                           double* out, int8_t* dphi, void* stream);
+
+/* ---------------------------------------------------------- foraging */
+/* One foraging replica (Simulation + Agent parameters, abm/simulation/sims.py:60-161,
+ * abm/contrib/decision_params.py:13-42, movement_params.py:13-24). */
+typedef struct abm_forage_replica {
+    int32_t n_agents;
+    int32_t n_rays;
+    int32_t visual_exclusion;     /* VISUAL_EXCLUSION                       */
+    int32_t patchwise_exclusion;  /* PATCHWISE_SOCIAL_EXCLUSION             */
+    int32_t tau;                  /* novelty ring length (DEC_TAU)          */
+    int32_t pad_;
+    double  fov0, fov1;
+    double  width, height, window_pad;
+    double  vision_range;
+    /* decision process */
+    double  T_w, Eps_w, g_w, B_w, w_max;
+    double  T_u, Eps_u, g_u, B_u, u_max;
+    double  S_wu, S_uw, F_N, F_R;
+    /* movement */
+    double  exp_vel_max, exp_theta_min, exp_theta_max;
+    double  reloc_theta_max, exp_stop_ratio;
+} abm_forage_replica;
+
+/* Agent modes as stored in abm_forage_state.mode (the reference's strings). */
+#define ABM_MODE_EXPLORE  0
+#define ABM_MODE_EXPLOIT  1
+#define ABM_MODE_RELOCATE 2
+#define ABM_MODE_COLLIDE  3
+#define ABM_MODE_POOL     4
+
+typedef struct abm_forage_batch abm_forage_batch;   /* opaque */
+
+/* Foraging agent state (SoA over all agents of all replicas).
+ * overriding_mode: -1 = None, else ABM_MODE_*.  novelty: tau doubles per agent
+ * (ring as a shifted array, index 0 newest).  is_moved_with_cursor is not
+ * modelled (always 0 in headless runs). */
+typedef struct abm_forage_state {
+    double* x; double* y; double* theta; double* v;
+    double* w; double* u; double* I_priv;
+    double* collected_r; double* collected_r_before;
+    double* novelty;
+    int32_t* mode; int32_t* overriding_mode;
+    int32_t* env_status; int32_t* exploited_patch_id;
+    const int32_t* radius;
+} abm_forage_state;
+
+int abm_forage_batch_create(const abm_forage_replica* replicas_host, int32_t n_replicas,
+                            abm_forage_batch** out);
+int abm_forage_batch_destroy(abm_forage_batch* batch);
+int abm_forage_batch_get_info(const abm_forage_batch* batch, abm_batch_info* out);
+int abm_forage_batch_get_offsets(const abm_forage_batch* batch, int32_t* agent_offset,
+                                 int64_t* field_offset);
+
+/* One Agent.update for every agent, reference order, float64.
+ * uniforms: per replica, a device array of the next numpy legacy-MT19937
+ *   doubles (random_sample) of the global stream, uniform_offset[b] its start
+ *   in the flat array and uniform_count[b] how many are available; the
+ *   random walk draws lo + (hi-lo)*u from it in agent order
+ *   (abm/agent/supcalc.py:38-47).
+ * uniforms_used[b] (device, int32): how many were consumed, so the host can
+ *   advance its global generator by exactly that many draws.
+ * field_bits (optional): soc_v_field in output order (bit k = soc_v_field[k]). */
+int abm_forage_step_f64(abm_forage_batch* batch, const abm_forage_state* state,
+                        const double* uniforms, const int64_t* uniform_offset,
+                        const int32_t* uniform_count, int32_t* uniforms_used,
+                        uint32_t* field_bits, void* stream);
+
+/* Agent.projection_field for one focal agent (abm/agent/agent.py:457-597).
+ * Obstacles are the social cues, followed by n_nonsocial non-social
+ * occluders when non_social >= 0 (the non_expl_agents argument); fov given
+ * explicitly.  Output: out_field[R] doubles (the returned array, after flip and
+ * FOV mask); keep_distance selects keep_distance_info. */
+int abm_forage_projection(int32_t n_rays, double fov0, double fov1, int32_t visual_exclusion,
+                          double vision_range, int32_t keep_distance,
+                          double fx, double fy, double ftheta, double fradius,
+                          const double* ox, const double* oy, int32_t n_social,
+                          int32_t n_nonsocial, double* out_field, void* stream);
+
+/* --------------------------------------------------------------- misc */
+const char* abm_last_error(void);
+/* Library version string and the gfx target it was built for. */
+const char* abm_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABM_AMD_H */
