@@ -176,6 +176,11 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
     bt->ray_total = ray_total;
     bt->n_tables = (int)table_off.size();
     int rc = check_hip(hipGetDevice(&bt->device), "hipGetDevice");
+    if (!rc) rc = check_hip(hipDeviceGetAttribute(&bt->dev_max_lds,
+                                                  hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                  bt->device), "hipDeviceGetAttribute");
+    if (!rc) rc = check_hip(hipDeviceGetAttribute(&bt->n_cu, hipDeviceAttributeMultiprocessorCount,
+                                                  bt->device), "hipDeviceGetAttribute");
     // tables
     std::vector<double> tc(ray_total), ts(ray_total), td(ray_total);
     std::vector<RayTab32> t32(ray_total);
@@ -283,7 +288,7 @@ int abm_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
 }
 
 int abm_vf_f32_fallback_count(abm_vf_batch* b, int64_t* pairs_total, int64_t* pairs_rechecked,
-                              void* stream) {
+                              int64_t* pairs_disagree, void* stream) {
     if (!b) return set_error(ABM_ERR_INVALID, "abm_vf_f32_fallback_count: NULL batch");
     unsigned long long h[4] = {0, 0, 0, 0};
     hipStream_t s = (hipStream_t)stream;
@@ -291,6 +296,7 @@ int abm_vf_f32_fallback_count(abm_vf_batch* b, int64_t* pairs_total, int64_t* pa
     if (int rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return rc;
     if (pairs_total) *pairs_total = (int64_t)h[0];
     if (pairs_rechecked) *pairs_rechecked = (int64_t)h[1];
+    if (pairs_disagree) *pairs_disagree = (int64_t)h[2];
     return ABM_OK;
 }
 

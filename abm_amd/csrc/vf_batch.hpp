@@ -48,6 +48,9 @@ struct abm_vf_batch {
     int32_t ray_total = 0;          // entries in each ray table
     int32_t waves_per_group = 0;    // fused kernel launch shape (0 = default)
     int device = 0;
+    int dev_max_lds = 0;            // device limits, read once at creation
+    int n_cu = 0;
+    int occ_f32[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [sync][nw 1,2,4]: groups per CU
     abm::VFRep* reps_dev = nullptr;
     double* tab_cos = nullptr;      // float64 tables: cos(PHI), sin(PHI), diff(PHI)
     double* tab_sin = nullptr;

@@ -180,14 +180,10 @@ int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bi
     if (b->max_rays >= 1024 || b->max_agents > 128) nthr = 128;
     if (b->max_agents > 512) nthr = 256;
     size_t lds = vf_f64_lds_bytes(b->max_agents, b->max_rays, nthr, sync_order);
-    int dev_max = 0;
-    if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                                 b->device), "hipDeviceGetAttribute"))
-        return rc;
-    if (lds > (size_t)dev_max)
+    if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f64: %d agents need %zu B of LDS per replica (device max %d)",
-                         b->max_agents, lds, dev_max);
+                         b->max_agents, lds, b->dev_max_lds);
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)vf_step_f64_kernel,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -159,13 +159,20 @@ class VFSwarm:
                  alp0=p(self.alp0), bet0=p(self.bet0), v0=p(self.v0))
 
     def step(self, order: str = "sequential", fields: bool = False, diag: bool = False,
-             stream=None):
-        """Advance every replica by one step (one kernel launch)."""
-        flags = L.ORDER_SEQUENTIAL if order == "sequential" else L.ORDER_SYNCHRONOUS
+             count: bool = False, verify: bool = False, stream=None):
+        """Advance every replica by one step (one kernel launch).
+
+        ``count`` / ``verify`` (fused kernel only) keep the cumulative pair,
+        float64 re-check and float32-vs-float64 disagreement counters."""
         if order not in ("sequential", "synchronous"):
             raise ValueError(order)
+        flags = L.ORDER_SEQUENTIAL if order == "sequential" else L.ORDER_SYNCHRONOUS
         if diag:
             flags |= L.WANT_DIAG
+        if count:
+            flags |= L.COUNT_RECHECKS
+        if verify:
+            flags |= L.VERIFY_FAST_PATH
         st = self._state()
         fb = L.ptr(self.fields) if fields else None
         dg = L.ptr(self.diag) if diag else None
@@ -174,10 +181,12 @@ class VFSwarm:
             L.check(fn(self.batch.handle, C.byref(st), fb, dg, flags, L.stream_ptr(stream)))
 
     def fallback_counts(self, stream=None):
-        tot, rec = C.c_int64(), C.c_int64()
+        """(pairs, float64 re-checks, float32/float64 disagreements), cumulative."""
+        tot, rec, dis = C.c_int64(), C.c_int64(), C.c_int64()
         L.check(self.batch.lib.abm_vf_f32_fallback_count(self.batch.handle, C.byref(tot),
-                                                         C.byref(rec), L.stream_ptr(stream)))
-        return tot.value, rec.value
+                                                         C.byref(rec), C.byref(dis),
+                                                         L.stream_ptr(stream)))
+        return tot.value, rec.value, dis.value
 
     def autotune(self, trial_steps=5, stream=None):
         if self.precision != "f32":

@@ -1,0 +1,239 @@
+"""Benchmark: batched VSWRM parameter sweep, fused float32 vs plain float64.
+
+Workload (one GPU): 4096 independent visual-flocking replicas on a 64 x 64
+grid of ALP0 x BET0 in [0.1, 3] x [0.1, 3]; N = 50 agents, R = 512 rays,
+radius 5, 707 x 707 periodic arena (density 1e-4), GAM 0.1, V0 1, ALP1 =
+BET1 = 0.09, reference (sequential) update order, per-replica seeds
+10000 + b, synthetic uniform start.  A step is one kernel launch advancing
+every replica by one simulation step (captured in a HIP graph and replayed).
+
+Prints ONE JSON line: the fused kernel's agent-steps/s (``value``), the plain
+float64 kernel's rate on the same workload, the one-step error between them,
+the roofline of the fused kernel and the CPU baseline (the oracle, one
+process per core, a bounded sample of the same replicas).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "agent-steps/s: batched FP32 vs plain FP64 GPU at fixed error; % VALU peak"
+B_SIDE, N, R, W = 64, 50, 512, 707.0
+PAD, RADIUS = 30.0, 5
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
+
+
+def ops_per_agent_step(n, r):
+    """Algorithmic lane-ops per agent-step: pair geometry 40 per other agent,
+    17 per ray, 20 per agent (transcendentals count 1)."""
+    return 40 * (n - 1) + 17 * r + 20
+
+
+def make_workload(n_rep_side=B_SIDE):
+    from abm_amd.vf import VFConfig
+    a = np.linspace(0.1, 3.0, n_rep_side)
+    cfgs, xs, ys, ths = [], [], [], []
+    for i in range(n_rep_side):
+        for j in range(n_rep_side):
+            b = i * n_rep_side + j
+            cfgs.append(VFConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=PAD,
+                                 boundary="infinite", GAM=0.1, V0=1.0, ALP0=float(a[i]),
+                                 ALP1=0.09, BET0=float(a[j]), BET1=0.09))
+            rng = np.random.RandomState(10000 + b)
+            xs.append(rng.uniform(PAD, PAD + W, N))
+            ys.append(rng.uniform(PAD, PAD + W, N))
+            ths.append(rng.uniform(0, 2 * np.pi, N))
+    f32 = lambda v: np.concatenate(v).astype(np.float32).astype(np.float64)
+    return cfgs, f32(xs), f32(ys), f32(ths)
+
+
+def time_steps(sw, steps, warmup, use_graph=True):
+    """Average kernel time (ms) of `steps` launches measured with events on
+    the launch stream, after `warmup` untimed steps."""
+    import torch
+    for _ in range(warmup):
+        sw.step()
+    torch.cuda.synchronize()
+    graph = None
+    if use_graph:
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            sw.step()                       # one eager launch on the capture stream
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            sw.step()
+        torch.cuda.synchronize()
+    run = graph.replay if graph is not None else sw.step
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        run()
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return float(np.mean(per)), float(np.median(per)), wall * 1e3 / steps
+
+
+# ------------------------------------------------------------ CPU baseline
+def _cpu_worker(args):
+    core, rep, steps = args
+    try:
+        os.sched_setaffinity(0, {core})
+    except (AttributeError, OSError):
+        pass
+    from oracle import vf_oracle as vo
+    a = np.linspace(0.1, 3.0, B_SIDE)
+    i, j = divmod(rep, B_SIDE)
+    rng = np.random.RandomState(10000 + rep)
+    x = rng.uniform(PAD, PAD + W, N)
+    y = rng.uniform(PAD, PAD + W, N)
+    th = rng.uniform(0, 2 * np.pi, N)
+    v = np.zeros(N)
+    p = vo.VFParams(GAM=0.1, V0=1.0, ALP0=float(a[i]), ALP1=0.09, BET0=float(a[j]), BET1=0.09,
+                    BOUNDARY="infinite")
+    ags = vo.make_agents(x, y, th, v, RADIUS, R, (-np.pi, np.pi), W, W, PAD, p)
+    vo.group_update(ags, p)                          # warm-up step
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        vo.group_update(ags, p)
+    dt = time.perf_counter() - t0
+    return steps * N / dt
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle on the host cores, one process per core, one replica each."""
+    cores = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    cores = cores[:16]
+    # size the sample: calibrate one step on one core
+    from oracle import vf_oracle as vo  # noqa: F401
+    steps = max(2, int(seconds / 0.12))
+    steps = min(steps, 60)
+    with mp.get_context("spawn").Pool(len(cores)) as pool:
+        rates = pool.map(_cpu_worker, [(c, k * 61 % (B_SIDE * B_SIDE), steps)
+                                       for k, c in enumerate(cores)])
+    return {"value": float(sum(rates)), "unit": "agent-steps/s", "cores": len(cores),
+            "kind": "port",
+            "sample": (f"oracle (numpy restatement of the reference step), {len(cores)} processes "
+                       f"pinned one per core, one replica each (N={N}, R={R}, torus) for "
+                       f"{steps} steps after 1 warm-up; per-core rate {np.mean(rates):.1f} "
+                       f"agent-steps/s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps-f64", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the replicas are independent and the workload "
+              f"is one GPU's share; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    # the CPU baseline runs first, in worker processes started before this
+    # process touches the GPU
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+
+    import torch
+    from abm_amd.vf import VFSwarm
+
+    cfgs, x, y, th = make_workload()
+    n_rep = len(cfgs)
+    agents = n_rep * N
+
+    # fixed-error check: one step of each kernel from the same state
+    sw64 = VFSwarm(cfgs, x, y, th, 0.0, radius=RADIUS, precision="f64")
+    sw32 = VFSwarm(cfgs, x, y, th, 0.0, radius=RADIUS, precision="f32")
+    sw64.step(fields=True)
+    sw32.step(fields=True, count=True)
+    torch.cuda.synchronize()
+    a64 = sw64.state_numpy()
+    a32 = sw32.state_numpy()
+    err = 0.0
+    for g, o, sc in zip(a32, a64, (1.0, 1.0, 1.0, 1.0)):
+        d = np.abs(g - o)
+        err = max(err, float(np.max(d / np.maximum(np.abs(o), sc))))
+    dth = np.abs((a32[2] - a64[2] + np.pi) % (2 * np.pi) - np.pi)
+    dx = np.abs((a32[0] - a64[0] + W / 2) % W - W / 2)
+    dy = np.abs((a32[1] - a64[1] + W / 2) % W - W / 2)
+    err = max(float(np.max(dx / np.maximum(np.abs(a64[0]), 1))),
+              float(np.max(dy / np.maximum(np.abs(a64[1]), 1))),
+              float(np.max(dth / np.maximum(np.abs(a64[2]), 1))),
+              float(np.max(np.abs(a32[3] - a64[3]) / np.maximum(np.abs(a64[3]), 1.0))))
+    fields_equal = bool(torch.equal(sw64.fields, sw32.fields))
+    pairs, rechecked, _ = sw32.fallback_counts()
+
+    # timing
+    ms32_mean, ms32_med, wall32 = time_steps(sw32, args.steps, args.warmup, not args.no_graph)
+    ms64_mean, ms64_med, wall64 = time_steps(sw64, args.steps_f64, min(args.warmup, 3),
+                                             not args.no_graph)
+    rate32 = agents / (ms32_mean * 1e-3)
+    rate64 = agents / (ms64_mean * 1e-3)
+    ops = ops_per_agent_step(N, R) * agents
+    achieved = ops / (ms32_mean * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": rate32,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32_mean,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"batched VSWRM sweep: {n_rep} replicas (64x64 ALP0 x BET0 in "
+                         f"[0.1,3]^2) x N={N} agents x R={R} rays, 707x707 periodic, "
+                         f"reference update order, fused float32 kernel under a HIP graph"),
+            "replicas": n_rep, "agents_per_replica": N, "rays": R,
+        },
+        "plain_f64": {"agent_steps_per_s": rate64, "ms_per_step": ms64_mean,
+                      "steps": args.steps_f64},
+        "speedup_vs_plain_f64": rate32 / rate64,
+        "one_step_rel_err_vs_f64": err,
+        "fields_equal_f32_vs_f64": fields_equal,
+        "f64_rechecked_pair_fraction": rechecked / max(pairs, 1),
+        "pair_ray_evals_per_s": agents * (N - 1) * R / (ms32_mean * 1e-3),
+        "ms_per_step_median": ms32_med,
+        "wall_ms_per_step": wall32,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": VALU_PEAK_TOPS,
+            "unit": "Tops/s",
+            "frac": achieved / VALU_PEAK_TOPS,
+            "traffic": None,
+            "ops_per_launch": ops,
+        },
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

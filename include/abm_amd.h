@@ -50,6 +50,9 @@ extern "C" {
 #define ABM_ORDER_SEQUENTIAL  0u  /* reference order: agent k sees agents <k moved      */
 #define ABM_ORDER_SYNCHRONOUS 1u  /* every field from the pre-step poses (not reference) */
 #define ABM_WANT_DIAG         2u  /* write per-agent (dv, dpsi, a_blob, a_edge, b_blob, b_edge) */
+#define ABM_COUNT_RECHECKS    4u  /* fused step: count pairs and float64 re-checks       */
+#define ABM_VERIFY_FAST_PATH  8u  /* fused step: also decide every pair in float64 and
+                                     count float32 decisions that differ (testing)     */
 
 /* ----------------------------------------------------- replica description */
 /*
@@ -133,10 +136,12 @@ int abm_vf_step_f64(abm_vf_batch* batch, const abm_vf_state_f64* state,
 int abm_vf_step_f32(abm_vf_batch* batch, const abm_vf_state_f32* state,
                     uint32_t* field_bits, float* diag, uint32_t flags, void* stream);
 
-/* Count of pair decisions that took the exact float64 re-check in the last
- * abm_vf_step_f32 on this batch (reads a device counter; synchronises). */
+/* Cumulative counters of the fused step since the batch was created, kept
+ * only by steps run with ABM_COUNT_RECHECKS / ABM_VERIFY_FAST_PATH: pairs
+ * decided, pairs re-decided in float64, and (verify mode) float32 decisions
+ * that differed from the float64 ones.  Synchronises the stream. */
 int abm_vf_f32_fallback_count(abm_vf_batch* batch, int64_t* pairs_total,
-                              int64_t* pairs_rechecked, void* stream);
+                              int64_t* pairs_rechecked, int64_t* pairs_disagree, void* stream);
 
 /* Select the fused kernel's launch shape: waves per workgroup for each size
  * class (0 = autotune result / default). */
