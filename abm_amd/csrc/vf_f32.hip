@@ -34,30 +34,36 @@ constexpr float kTwoPiF = 6.28318530717958647692f;
 constexpr float kHalfPiF = 1.57079632679489661923f;
 constexpr float kThreeHalfPiF = 4.71238898038468985769f;
 
-// Guard bands.  Float32 angle error (sincos of the heading, cross/dot,
-// atan2f) is below 1e-6 rad and the float64 reference angle is within 5e-8
-// rad of the exact one, so 4e-6 rad covers both with margin; the half-width
-// guard is relative (atanf, the division and sqrt: < 1e-6 relative).
-constexpr double kAngleGuard = 4e-6;
-constexpr double kHalfGuardRel = 4e-6;
-constexpr double kHalfGuardAbs = 1e-6;
+// Guard bands.  The float32 angle (sincos of the heading, centre
+// differences, cross/dot, atan2_f32) is within 1e-6 rad of the exact one and
+// the float64 reference angle within 5e-8 rad, so a 4e-6 rad band around
+// every bin tie and around the +-pi wrap covers both with margin.  The
+// half-width band is 4e-6 relative + 1e-6 (its float32 error is < 1e-6
+// relative).  Pairs inside a band are re-decided in float64.
+constexpr float kAngleGuardF = 4e-6f;
 
-// Cross-lane sum of a double over the wave with DPP moves only (no LDS):
-// quad xor 1, xor 2, half-row mirror, row mirror, then row_bcast15 into rows
-// 1 and 3 and row_bcast31 into rows 2 and 3; lane 63 ends with the total.
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ inline double dpp_f64(double v) {
+// Wave sum: float32 DPP adds inside each 16-lane row (quad xor 1, xor 2,
+// half-row mirror, row mirror), then float64 across rows (row_bcast15 into
+// rows 1 and 3, row_bcast31 into rows 2 and 3); lane 63 ends with the total,
+// returned as a wave-uniform double.
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64_masked(double v) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
     return __hiloint2double(hi, lo);
 }
-__device__ inline double wave_total_f64(double v) {
-    v += dpp_f64<0xB1>(v);         // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);         // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);        // row_half_mirror
-    v += dpp_f64<0x140>(v);        // row_mirror
-    v += dpp_f64<0x142, 0xa>(v);   // row_bcast15 -> rows 1, 3
-    v += dpp_f64<0x143, 0xc>(v);   // row_bcast31 -> rows 2, 3
+__device__ inline double wave_total(float x) {
+    x += dpp_f32<0xB1>(x);          // quad_perm [1,0,3,2]
+    x += dpp_f32<0x4E>(x);          // quad_perm [2,3,0,1]
+    x += dpp_f32<0x141>(x);         // row_half_mirror
+    x += dpp_f32<0x140>(x);         // row_mirror: every lane holds its row's sum
+    double v = (double)x;
+    v += dpp_f64_masked<0x142, 0xa>(v);   // row_bcast15 -> rows 1, 3
+    v += dpp_f64_masked<0x143, 0xc>(v);   // row_bcast31 -> rows 2, 3
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
@@ -83,38 +89,142 @@ __device__ inline void or_range(uint32_t* F, int a, int b) {
     }
 }
 
+__device__ inline void raster(uint32_t* F, int start, int end, int R) {
+    int p0, p1, q0, q1;
+    vf_pieces(start, end, R, &p0, &p1, &q0, &q1);
+    or_range(F, p0, p1);
+    or_range(F, q0, q1);
+}
+
 __device__ inline float wrap_2pi(float o) {
     if (o < 0.f) o = kTwoPiF + o;
     if (o > kTwoPiF) o = o - kTwoPiF;
     return o;
 }
 
+// atan2 for float32 from a degree-7 minimax polynomial of atan(z)/z in z^2 on
+// [0, 1] (|error| <= 1.5e-7 with fused multiply-adds) and octant reduction;
+// total error below 4e-7 rad.  The guard bands allow for it.
+__device__ inline float atan2_f32(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float z = mn * __builtin_amdgcn_rcpf(mx);
+    const float s = z * z;
+    float p = -0.0046934f;
+    p = fmaf(p, s, 0.02425289f);
+    p = fmaf(p, s, -0.05948712f);
+    p = fmaf(p, s, 0.09914348f);
+    p = fmaf(p, s, -0.14019503f);
+    p = fmaf(p, s, 0.19969729f);
+    p = fmaf(p, s, -0.3333199f);
+    p = fmaf(p, s, 0.9999999f);
+    p = p * z;
+    if (ay > ax) p = kHalfPiF - p;
+    if (x < 0.f) p = kPiF - p;
+    return y < 0.f ? -p : p;
+}
+
+// Per-replica constants, all float32 except the few integers.
+struct RepConst {
+    int N, R, W, a0, torus, fov_px0, fov_px1, phi_ok, limit;
+    long field_offset;
+    float inv_step;   // (R - 1) / (2 pi): bins per radian of linspace(-pi, pi, R)
+    float g_bins;     // bin-tie guard in bins (angle guard + float32 rounding of the coordinate)
+    float hs;         // R / (2 pi): proj_size / 2 per radian of atan(size / d)
+    float hw, hh;     // W/2, H/2
+    float tguard;     // guard on |dx| - W/2 for the torus image decision
+    float fW, fH, pad;
+    float gam, alp1, bet1, max_th, max_vel;
+};
+
+__device__ inline RepConst rep_const(const VFRep* rp) {
+    RepConst c;
+    c.N = rp->n_agents; c.R = rp->n_rays; c.W = rp->words; c.a0 = rp->agent_offset;
+    c.torus = rp->boundary == ABM_BOUNDARY_INFINITE;
+    c.fov_px0 = rp->fov_px0; c.fov_px1 = rp->fov_px1; c.phi_ok = rp->phi_ok;
+    c.limit = rp->limit_movement; c.field_offset = rp->field_offset;
+    c.inv_step = (float)(1.0 / rp->step);
+    c.g_bins = (float)(kAngleGuardF / rp->step + 1.5e-7 * rp->n_rays);
+    c.hs = (float)((double)rp->n_rays / kTwoPi);
+    c.hw = (float)(rp->width / 2.0);
+    c.hh = (float)(rp->height / 2.0);
+    c.tguard = (float)(1e-6 * (rp->width > rp->height ? rp->width : rp->height) + 1e-4);
+    c.fW = (float)rp->width; c.fH = (float)rp->height; c.pad = (float)rp->pad;
+    c.gam = (float)rp->gam; c.alp1 = (float)rp->alp1; c.bet1 = (float)rp->bet1;
+    c.max_th = (float)rp->max_th; c.max_vel = (float)rp->max_vel;
+    return c;
+}
+
+// Float32 decision of one pair; returns false when the pair must be
+// re-decided in float64 (inside a guard band, or degenerate).  Centre
+// differences of float32 positions are exact or within 1 ulp of the result
+// (Sterbenz), so the angle is within 1e-6 rad of the float64 one.
+__device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
+                                 float ojy, float ojr, const RepConst& C, int* start, int* end,
+                                 int* drawn) {
+    float dx = (ojx - fx) + (ojr - fr);
+    float dy = (ojy - fy) + (ojr - fr);
+    bool ok = true;
+    if (C.torus) {
+        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+        ok = fabsf(ex) >= C.tguard && fabsf(ey) >= C.tguard;
+        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+    }
+    const float d2 = fmaf(dy, dy, dx * dx);
+    const float d = __builtin_amdgcn_sqrtf(d2);
+    const float a = atan2_f32(fmaf(u1x, dy, -u1y * dx), fmaf(u1x, dx, u1y * dy));
+    const float q = atan2_f32(ojr, d);
+    const float xb = (kPiF - a) * C.inv_step;       // bin coordinate of ca = -a
+    const float xfl = floorf(xb);
+    const float fb = xb - xfl;
+    const float half = q * C.hs;                    // proj_size / 2
+    const float hfl = floorf(half);
+    const float hfrac = half - hfl;
+    const float hg = 4e-6f * half + 1e-6f;
+    ok = ok && (d2 > 1e-30f) && fabsf(a) < kPiF - kAngleGuardF && fabsf(fb - 0.5f) >= C.g_bins &&
+         hfrac >= hg && hfrac <= 1.0f - hg;
+    int t = (int)xfl + (fb >= 0.5f ? 1 : 0);
+    t = t < 0 ? 0 : (t > C.R - 1 ? C.R - 1 : t);
+    const int h = (int)hfl;
+    *start = t - h;
+    *end = t + h;
+    *drawn = ((C.fov_px0 < *start && *start < C.fov_px1) ||
+              (C.fov_px0 < *end && *end < C.fov_px1)) ? 1 : 0;
+    return ok;
+}
+
+__device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx, float ojy,
+                                      float ojr, const VFRep* rp, int* start, int* end, int* drawn) {
+    const Focal f = make_focal((double)fx, (double)fy, (double)fth, (double)fr);
+    const PairVF e = pair_vf_exact(f, (double)ojx, (double)ojy, (double)ojr,
+                                   rp->boundary == ABM_BOUNDARY_INFINITE ? 1 : 0, rp->width,
+                                   rp->height, rp->n_rays, rp->step, rp->fov_px0, rp->fov_px1);
+    *start = e.start; *end = e.end; *drawn = e.skipped ? 0 : e.drawn;
+}
+
+// LDS layout: per-agent records as float4 so one 16-byte read serves a pair.
 struct Smem {
-    RayTab32* tab;
-    float *px, *py, *th, *vv, *rad, *ox, *oy;
-    int32_t* rlist;     // pairs queued for the float64 decision
-    int32_t* rfast;     // their float32 decision (verify mode)
-    uint32_t* F;
-    double* red;
+    RayTab32* tab;     // transposed ray table: entry i*NT + lane holds ray lane*RPL + i
+    float4* pose;      // live (x, y, radius, theta) of every agent
+    float4* obj;       // positions the pairs read: = pose, or a pre-step copy (synchronous)
+    float4* dyn;       // (v, ALP0, BET0, V0)
+    int32_t* rlist;    // pairs queued for the float64 decision
+    int32_t* rfast;    // their float32 decision (verify mode)
+    uint32_t* F;       // the focal agent's bit-packed field
+    double* red;       // cross-wave partial sums
     int32_t* nre;
 };
 
-__device__ inline Smem carve(void* base, int maxN, int maxR, int nw, bool sync_order) {
+__device__ inline Smem carve(void* base, int maxN, int tab_entries, int nw, bool sync_order) {
     Smem s;
     char* p = reinterpret_cast<char*>(base);
-    s.tab = reinterpret_cast<RayTab32*>(p); p += sizeof(RayTab32) * maxR;
-    s.red = reinterpret_cast<double*>(p); p += sizeof(double) * (4 * nw + 8);
-    s.px = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    s.py = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    s.th = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    s.vv = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    s.rad = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    if (sync_order) {
-        s.ox = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-        s.oy = reinterpret_cast<float*>(p); p += sizeof(float) * maxN;
-    } else {
-        s.ox = s.px; s.oy = s.py;
-    }
+    s.tab = reinterpret_cast<RayTab32*>(p); p += sizeof(RayTab32) * tab_entries;
+    s.pose = reinterpret_cast<float4*>(p); p += sizeof(float4) * maxN;
+    s.dyn = reinterpret_cast<float4*>(p); p += sizeof(float4) * maxN;
+    if (sync_order) { s.obj = reinterpret_cast<float4*>(p); p += sizeof(float4) * maxN; }
+    else s.obj = s.pose;
+    s.red = reinterpret_cast<double*>(p); p += sizeof(double) * (2 * nw + 2);
     s.rlist = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * maxN;
     s.rfast = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * maxN;
     s.nre = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * 4;
@@ -122,133 +232,77 @@ __device__ inline Smem carve(void* base, int maxN, int maxR, int nw, bool sync_o
     return s;
 }
 
-// Float32 decision of one pair; returns false when the pair must be
-// re-decided in float64 (inside a guard band, or degenerate).
-__device__ inline bool fast_pair(float u1x, float u1y, double cx, double cy, float ojx, float ojy,
-                                 float ojr, bool torus, const VFRep& P, double inv_step,
-                                 double half_scale, int R, int* start, int* end, int* drawn) {
-    double ocx = (double)ojx + (double)ojr;
-    double ocy = (double)ojy + (double)ojr;
-    double dx = ocx - cx, dy = ocy - cy;
-    if (torus) {
-        if (fabs(dx) > P.width / 2.0) {
-            if (cx < ocx) ocx -= P.width;
-            else if (cx > ocx) ocx += P.width;
-        }
-        if (fabs(dy) > P.height / 2.0) {
-            if (cy < ocy) ocy -= P.height;
-            else if (cy > ocy) ocy += P.height;
-        }
-        dx = ocx - cx;
-        dy = ocy - cy;
-    }
-    const float fdx = (float)dx, fdy = (float)dy;
-    const float d = sqrtf(fdx * fdx + fdy * fdy);
-    const float a = atan2f(u1x * fdy - u1y * fdx, u1x * fdx + u1y * fdy);
-    const float q = atanf(ojr / d);
-    const double xb = (-(double)a + kPi) * inv_step;   // bin coordinate of ca = -a
-    const double fl = floor(xb);
-    const double fb = xb - fl;
-    const double half = (double)q * half_scale;        // proj_size / 2
-    const double hfl = floor(half);
-    const double hfrac = half - hfl;
-    const double hg = kHalfGuardRel * half + kHalfGuardAbs;
-    const bool ok = (d > 0.0f) && fabs((double)a) < kPi - kAngleGuard &&
-                    fabs(fb - 0.5) >= kAngleGuard * inv_step && hfrac >= hg && hfrac <= 1.0 - hg;
-    int t = (int)fl + (fb >= 0.5 ? 1 : 0);
-    t = t < 0 ? 0 : (t > R - 1 ? R - 1 : t);
-    const int h = (int)hfl;
-    *start = t - h;
-    *end = t + h;
-    *drawn = ((P.fov_px0 < *start && *start < P.fov_px1) ||
-              (P.fov_px0 < *end && *end < P.fov_px1)) ? 1 : 0;
-    return ok;
-}
-
-__device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx, float ojy,
-                                        float ojr, bool torus, const VFRep& P, int* start, int* end,
-                                        int* drawn) {
-    const Focal f = make_focal((double)fx, (double)fy, (double)fth, (double)fr);
-    const PairVF e = pair_vf_exact(f, (double)ojx, (double)ojy, (double)ojr, torus ? 1 : 0,
-                                   P.width, P.height, P.n_rays, P.step, P.fov_px0, P.fov_px1);
-    *start = e.start; *end = e.end; *drawn = e.skipped ? 0 : e.drawn;
-}
-
 template <int NW, int RPL>
 __global__ void __launch_bounds__(64 * NW)
 vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order, int n_rep,
                    const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
                    uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
-                   unsigned long long* __restrict__ counters, int maxN, int maxR) {
+                   unsigned long long* __restrict__ counters, int maxN) {
     extern __shared__ __align__(16) double smem_d[];
     constexpr int NT = 64 * NW;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
     const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & ABM_COUNT_RECHECKS) != 0 && counters != nullptr;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
     const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
-    Smem S = carve(smem_d, maxN, maxR, NW, sync_order);
+    Smem S = carve(smem_d, maxN, RPL * NT, NW, sync_order);
     int loaded_tab = -1;
     unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
 
     for (int idx = blockIdx.x; idx < n_rep; idx += gridDim.x) {
-        const VFRep P = reps[order[idx]];
-        const int N = P.n_agents, R = P.n_rays, W = P.words, a0 = P.agent_offset;
-        const bool torus = P.boundary == ABM_BOUNDARY_INFINITE;
-        if (P.table_offset != loaded_tab) {
-            for (int r = tid; r < R; r += NT) S.tab[r] = tab32[P.table_offset + r];
-            loaded_tab = P.table_offset;
+        const VFRep* rp = reps + order[idx];
+        const RepConst C = rep_const(rp);
+        const int tab_off = rp->table_offset;
+        if (tab_off != loaded_tab) {
+            for (int e = tid; e < RPL * NT; e += NT) {
+                const int r = (e % NT) * RPL + e / NT;
+                RayTab32 z;
+                z.wcos = z.wsin = z.cos = z.sin = 0.f;
+                S.tab[e] = r < C.R ? tab32[tab_off + r] : z;
+            }
+            loaded_tab = tab_off;
         }
-        for (int i = tid; i < N; i += NT) {
-            S.px[i] = st.x[a0 + i];
-            S.py[i] = st.y[a0 + i];
-            S.th[i] = st.theta[a0 + i];
-            S.vv[i] = st.v[a0 + i];
-            S.rad[i] = (float)st.radius[a0 + i];
-            if (sync_order) { S.ox[i] = S.px[i]; S.oy[i] = S.py[i]; }
+        const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
+        for (int i = tid; i < C.N; i += NT) {
+            const int g = C.a0 + i;
+            const float4 p = make_float4(st.x[g], st.y[g], (float)st.radius[g], st.theta[g]);
+            S.pose[i] = p;
+            if (sync_order) S.obj[i] = p;
+            S.dyn[i] = make_float4(st.v[g], st.alp0 ? st.alp0[g] : a0d, st.bet0 ? st.bet0[g] : b0d,
+                                   st.v0 ? st.v0[g] : v0d);
         }
-        const double inv_step = 1.0 / P.step;
-        const double half_scale = (double)R / kTwoPi;
-        const float fW = (float)P.width, fH = (float)P.height, fpad = (float)P.pad;
-        const float gam = (float)P.gam, alp1 = (float)P.alp1, bet1 = (float)P.bet1;
         const int rs = tid * RPL;            // this lane's ray span [rs, rs + rn)
-        const int rn = rs >= R ? 0 : (R - rs < RPL ? R - rs : RPL);
+        const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
         if (tid == 0) S.nre[0] = 0;
         __syncthreads();
         // heading of agent 0 (later headings come with the previous agent's motion)
         float sth, cth;
-        sincosf(S.th[0], &sth, &cth);
+        sincosf(S.pose[0].w, &sth, &cth);
 
-        for (int k = 0; k < N; ++k) {
-            for (int w = tid; w < W; w += NT) S.F[w] = 0u;
-            const float fx = S.px[k], fy = S.py[k], fth = S.th[k], fr = S.rad[k];
+        for (int k = 0; k < C.N; ++k) {
+            for (int w = tid; w < C.W; w += NT) S.F[w] = 0u;
+            const float4 fp = S.pose[k];
+            const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
             const float u1x = cth, u1y = -sth;           // unit(v1) = (cos th, -sin th)
-            const double cx = (double)fx + (double)fr;
-            const double cy = (double)fy + (double)fr;
             __syncthreads();
 
             // ------------------------------------------------ pairs, float32
-            for (int j = tid; j < N; j += NT) {
+            for (int j = tid; j < C.N; j += NT) {
                 if (j == k) continue;
-                const float ojx = S.ox[j], ojy = S.oy[j], ojr = S.rad[j];
-                if (ojx == fx && ojy == fy) continue;          // coincident (:57)
+                const float4 o = S.obj[j];
+                if (o.x == fx && o.y == fy) continue;          // coincident (:57)
                 int start, end, drawn;
-                const bool ok = fast_pair(u1x, u1y, cx, cy, ojx, ojy, ojr, torus, P, inv_step,
-                                          half_scale, R, &start, &end, &drawn);
-                if (count || verify) ++n_pairs;
+                const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end,
+                                          &drawn);
+                if (count) ++n_pairs;
                 if (!ok || verify) {
                     const int slot = atomicAdd(S.nre, 1);
                     S.rlist[slot] = ok ? j : (j | (1 << 30));
-                    S.rfast[slot] = drawn ? ((start + 2 * R) | ((end + 2 * R) << 15)) : -1;
+                    S.rfast[slot] = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
                     if (!ok) continue;
                 }
-                if (drawn) {
-                    int p0, p1, q0, q1;
-                    vf_pieces(start, end, R, &p0, &p1, &q0, &q1);
-                    or_range(S.F, p0, p1);
-                    or_range(S.F, q0, q1);
-                }
+                if (drawn) raster(S.F, start, end, C.R);
             }
             __syncthreads();
             // ------------------------------------- pairs re-decided in float64
@@ -258,21 +312,15 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
                     const int e = S.rlist[i];
                     const int j = e & ((1 << 30) - 1);
                     const bool forced = (e >> 30) & 1;
+                    const float4 o = S.obj[j];
                     int start, end, drawn;
-                    exact_pair(fx, fy, fth, fr, S.ox[j], S.oy[j], S.rad[j], torus, P, &start,
-                               &end, &drawn);
+                    exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &start, &end, &drawn);
                     if (forced) {
-                        if (count || verify) ++n_recheck;
-                        if (drawn) {
-                            int p0, p1, q0, q1;
-                            vf_pieces(start, end, R, &p0, &p1, &q0, &q1);
-                            or_range(S.F, p0, p1);
-                            or_range(S.F, q0, q1);
-                        }
+                        if (count) ++n_recheck;
+                        if (drawn) raster(S.F, start, end, C.R);
                     } else {
-                        const int fast = S.rfast[i];
-                        const int ex = drawn ? ((start + 2 * R) | ((end + 2 * R) << 15)) : -1;
-                        if (fast != ex) ++n_disagree;
+                        const int ex = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
+                        if (S.rfast[i] != ex) ++n_disagree;
                     }
                 }
                 __syncthreads();
@@ -280,92 +328,92 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
             }
 
             // ------------------------------------------------ sums
-            const uint32_t v0bit = S.F[0] & 1u;
-            const uint32_t vlast = field_bit(S.F, R - 1);
-            const bool case1 = v0bit == 1u && vlast == 0u;
+            const bool case1 = (S.F[0] & 1u) == 1u && field_bit(S.F, C.R - 1) == 0u;
             float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
-            if (rn > 0) {
+            {
+                // rays at or beyond R have zero table entries and zero bits
                 const int w0 = rs >> 5;
-                const uint32_t lo = S.F[w0];
-                const uint32_t hi = (RPL > 1 && w0 + 1 < W) ? S.F[w0 + 1] : 0u;
+                const uint32_t lo = w0 < C.W ? S.F[w0] : 0u;
+                const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? S.F[w0 + 1] : 0u;
                 const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
                 const uint32_t m = low_mask(rn);
                 const uint32_t cur = (uint32_t)v64 & m;
-                uint32_t spike;
-                if (case1) {
-                    const uint32_t prev = field_bit(S.F, rs == 0 ? R - 1 : rs - 1);
-                    spike = (cur ^ ((cur << 1) | prev)) & m;
-                } else {
-                    const int nx = rs + rn;
-                    const uint32_t next = field_bit(S.F, nx >= R ? 0 : nx);
-                    spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+                uint32_t spike = 0u;
+                if (rn > 0) {
+                    if (case1) {
+                        const uint32_t prev = field_bit(S.F, rs == 0 ? C.R - 1 : rs - 1);
+                        spike = (cur ^ ((cur << 1) | prev)) & m;
+                    } else {
+                        const int nx = rs + rn;
+                        const uint32_t next = field_bit(S.F, nx >= C.R ? 0 : nx);
+                        spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < RPL; ++i) {
-                    if (i < rn) {
-                        const RayTab32 T = S.tab[rs + i];
-                        const bool vb = (cur >> i) & 1u;
-                        const bool sb = (spike >> i) & 1u;
-                        bc += vb ? T.wcos : 0.f;
-                        bs += vb ? T.wsin : 0.f;
-                        ec += sb ? T.cos : 0.f;
-                        es += sb ? T.sin : 0.f;
-                    }
+                    const RayTab32 T = S.tab[i * NT + tid];
+                    const bool vb = (cur >> i) & 1u;
+                    const bool sb = (spike >> i) & 1u;
+                    bc += vb ? T.wcos : 0.f;
+                    bs += vb ? T.wsin : 0.f;
+                    ec += sb ? T.cos : 0.f;
+                    es += sb ? T.sin : 0.f;
                 }
             }
-            const int ag = a0 + k;
-            const float alp0 = st.alp0 ? st.alp0[ag] : (float)P.alp0;
-            const float bet0 = st.bet0 ? st.bet0[ag] : (float)P.bet0;
-            const float v0 = st.v0 ? st.v0[ag] : (float)P.v0;
+            const float4 dk = S.dyn[k];
+            const float alp0 = dk.y, bet0 = dk.z, v0 = dk.w;
             // blob terms: trapz(cos(PHI) * (-V)) = -sum(w cos V)
             double Sv, Sp, Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
             if (!want_diag) {
-                Sv = wave_total_f64((double)(alp0 * (-bc) + (alp0 * alp1) * ec));
-                Sp = wave_total_f64((double)(bet0 * (-bs) + (bet0 * bet1) * es));
+                Sv = wave_total(alp0 * (-bc) + (alp0 * C.alp1) * ec);
+                Sp = wave_total(bet0 * (-bs) + (bet0 * C.bet1) * es);
             } else {
-                Bc = wave_total_f64((double)bc); Bs = wave_total_f64((double)bs);
-                Ec = wave_total_f64((double)ec); Es = wave_total_f64((double)es);
-                Sv = (double)alp0 * -Bc + (double)(alp0 * alp1) * Ec;
-                Sp = (double)bet0 * -Bs + (double)(bet0 * bet1) * Es;
+                Bc = wave_total(bc); Bs = wave_total(bs);
+                Ec = wave_total(ec); Es = wave_total(es);
+                Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+                Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
             }
             if (NW > 1) {
                 if (lane == 0) {
-                    double* rr = S.red + 6 * (tid >> 6);
-                    rr[0] = Sv; rr[1] = Sp; rr[2] = Bc; rr[3] = Bs; rr[4] = Ec; rr[5] = Es;
+                    S.red[2 * (tid >> 6)] = Sv; S.red[2 * (tid >> 6) + 1] = Sp;
                 }
                 __syncthreads();
-                Sv = Sp = Bc = Bs = Ec = Es = 0.0;
-                for (int w = 0; w < NW; ++w) {
-                    const double* rr = S.red + 6 * w;
-                    Sv += rr[0]; Sp += rr[1]; Bc += rr[2]; Bs += rr[3]; Ec += rr[4]; Es += rr[5];
+                Sv = Sp = 0.0;
+                for (int w = 0; w < NW; ++w) { Sv += S.red[2 * w]; Sp += S.red[2 * w + 1]; }
+                if (want_diag) {   // components of the other waves
+                    __syncthreads();
+                    double comp[4] = {Bc, Bs, Ec, Es};
+                    for (int c = 0; c < 4; ++c) {
+                        if (lane == 0) S.red[tid >> 6] = comp[c];
+                        __syncthreads();
+                        double t = 0.0;
+                        for (int w = 0; w < NW; ++w) t += S.red[w];
+                        comp[c] = t;
+                        __syncthreads();
+                    }
+                    Bc = comp[0]; Bs = comp[1]; Ec = comp[2]; Es = comp[3];
                 }
             }
             if (bits_out) {
-                uint32_t* dst = bits_out + P.field_offset + (int64_t)k * W;
-                for (int w = tid; w < W; w += NT) dst[w] = S.F[w];
+                uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
+                for (int w = tid; w < C.W; w += NT) dst[w] = S.F[w];
             }
 
             // ------------------------------------------------ motion
-            float vel = S.vv[k];
+            float vel = dk.x;
             float dv = 0.f, dpsi = 0.f;
-            if (P.phi_ok) {
-                dv = gam * (v0 - vel) + (float)Sv;
+            if (C.phi_ok) {
+                dv = C.gam * (v0 - vel) + (float)Sv;
                 dpsi = (float)Sp;
             }
             float theta = dpsi;
-            if (P.limit_movement) {
-                const float lim = (float)P.max_th;
-                if (fabsf(theta) > lim) theta = theta < 0.f ? -lim : lim;
-            }
+            if (C.limit && fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
             float o = wrap_2pi(fth + theta);
             vel = vel + dv;
-            if (P.limit_movement) {
-                const float lim = (float)P.max_vel;
-                if (fabsf(vel) > lim) vel = vel < 0.f ? -lim : lim;
-            }
+            if (C.limit && fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
             // one sincos serves this agent's motion (lane 0) and the next
             // agent's heading (lane 1)
-            const float ang = (lane == 1 && k + 1 < N) ? S.th[k + 1] : o;
+            const float ang = (lane == 1 && k + 1 < C.N) ? S.pose[k + 1].w : o;
             float s_l, c_l;
             sincosf(ang, &s_l, &c_l);
             const float so = lane_f32(s_l, 0), co = lane_f32(c_l, 0);
@@ -374,70 +422,74 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
             float nx = fx + vel * co;
             float ny = fy - vel * so;
             const float xc = nx + fr, yc = ny + fr;
-            if (torus) {
-                if (xc < fpad) nx = fpad + fW - fr;
-                else if (xc > fpad + fW) nx = fpad + fr;
-                if (yc < fpad) ny = fpad + fH - fr;
-                else if (yc > fpad + fH) ny = fpad + fr;
+            const float lo_b = C.pad, hx_b = C.pad + C.fW, hy_b = C.pad + C.fH;
+            if (C.torus) {
+                if (xc < lo_b) nx = hx_b - fr;
+                else if (xc > hx_b) nx = lo_b + fr;
+                if (yc < lo_b) ny = hy_b - fr;
+                else if (yc > hy_b) ny = lo_b + fr;
             } else {
-                if (xc < fpad) {
-                    nx = fpad - fr;
+                if (xc < lo_b) {
+                    nx = lo_b - fr;
                     if (kHalfPiF <= o && o < kPiF) o -= kHalfPiF;
                     else if (kPiF <= o && o <= kThreeHalfPiF) o += kHalfPiF;
                     o = wrap_2pi(o);
                 }
-                if (xc > fpad + fW) {
-                    nx = fpad + fW - fr - 1.f;
+                if (xc > hx_b) {
+                    nx = hx_b - fr - 1.f;
                     if (kThreeHalfPiF <= o && o < kTwoPiF) o -= kHalfPiF;
                     else if (0.f <= o && o <= kHalfPiF) o += kHalfPiF;
                     o = wrap_2pi(o);
                 }
-                if (yc < fpad) {
-                    ny = fpad - fr;
+                if (yc < lo_b) {
+                    ny = lo_b - fr;
                     if (kHalfPiF <= o && o <= kPiF) o += kHalfPiF;
                     else if (0.f <= o && o < kHalfPiF) o -= kHalfPiF;
                     o = wrap_2pi(o);
                 }
-                if (yc > fpad + fH) {
-                    ny = fpad + fH - fr - 1.f;
+                if (yc > hy_b) {
+                    ny = hy_b - fr - 1.f;
                     if (kThreeHalfPiF <= o && o <= kTwoPiF) o += kHalfPiF;
                     else if (kPiF <= o && o < kThreeHalfPiF) o -= kHalfPiF;
                     o = wrap_2pi(o);
                 }
             }
             if (tid == 0) {
-                S.px[k] = nx; S.py[k] = ny; S.th[k] = o; S.vv[k] = vel;
+                S.pose[k] = make_float4(nx, ny, fr, o);
+                S.dyn[k].x = vel;
                 if (want_diag) {
-                    float* dd = diag + 6 * (int64_t)ag;
+                    float* dd = diag + 6 * (int64_t)(C.a0 + k);
                     dd[0] = dv; dd[1] = dpsi;
-                    dd[2] = P.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
-                    dd[3] = P.phi_ok ? (float)((double)(alp0 * alp1) * Ec) : 0.f;
-                    dd[4] = P.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
-                    dd[5] = P.phi_ok ? (float)((double)(bet0 * bet1) * Es) : 0.f;
+                    dd[2] = C.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
+                    dd[3] = C.phi_ok ? (float)((double)(alp0 * C.alp1) * Ec) : 0.f;
+                    dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
+                    dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
                 }
             }
             __syncthreads();
         }
-        for (int i = tid; i < N; i += NT) {
-            st.x[a0 + i] = S.px[i];
-            st.y[a0 + i] = S.py[i];
-            st.theta[a0 + i] = S.th[i];
-            st.v[a0 + i] = S.vv[i];
+        for (int i = tid; i < C.N; i += NT) {
+            const int g = C.a0 + i;
+            const float4 p = S.pose[i];
+            st.x[g] = p.x;
+            st.y[g] = p.y;
+            st.theta[g] = p.w;
+            st.v[g] = S.dyn[i].x;
         }
         __syncthreads();
     }
-    if (count || verify) {
+    if (count) {
         if (n_pairs) atomicAdd(&counters[0], n_pairs);
         if (n_recheck) atomicAdd(&counters[1], n_recheck);
         if (n_disagree) atomicAdd(&counters[2], n_disagree);
     }
 }
 
-size_t lds_bytes_f32(int maxN, int maxR, int nw, bool sync_order) {
+size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
     const int maxW = (maxR + 31) >> 5;
-    size_t b = sizeof(RayTab32) * (size_t)maxR + sizeof(double) * (4 * nw + 8) +
-               sizeof(float) * (size_t)maxN * (sync_order ? 7 : 5) +
-               sizeof(int32_t) * ((size_t)2 * maxN + 4) + sizeof(uint32_t) * (maxW + 2);
+    size_t b = sizeof(RayTab32) * (size_t)rpl * 64 * nw + sizeof(float4) * (size_t)maxN * (sync_order ? 3 : 2) +
+               sizeof(double) * (2 * nw + 2) + sizeof(int32_t) * ((size_t)2 * maxN + 4) +
+               sizeof(uint32_t) * (maxW + 2);
     return (b + 15) & ~(size_t)15;
 }
 
@@ -445,7 +497,7 @@ template <int NW, int RPL>
 int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
               uint32_t flags, hipStream_t s) {
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
-    const size_t lds = lds_bytes_f32(b->max_agents, b->max_rays, NW, sync_order);
+    const size_t lds = lds_bytes_f32(b->max_agents, b->max_rays, NW, RPL, sync_order);
     if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: %d agents x %d rays need %zu B of LDS (device max %d)",
@@ -471,7 +523,7 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
     if (grid > b->n_replicas) grid = b->n_replicas;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NW), lds, s, b->reps_dev, b->order_dev,
                        b->n_replicas, b->tab32, *st, bits, diag, flags, b->counters,
-                       b->max_agents, b->max_rays);
+                       b->max_agents);
     return check_hip(hipGetLastError(), "vf_step_f32_kernel launch");
 }
 
