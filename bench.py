@@ -90,6 +90,18 @@ def time_steps(sw, steps, warmup, use_graph=True):
     return float(np.mean(per)), float(np.median(per)), wall * 1e3 / steps
 
 
+def traffic_bytes():
+    """HBM bytes per launch of the fused kernel on this workload, as measured by
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a device copy of known size)
+    and recorded next to the profile summaries; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_vswrm_sweep_4096x50x512.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["bytes_per_launch"]["f32"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 # ------------------------------------------------------------ CPU baseline
 def _cpu_worker(args):
     core, rep, steps = args
@@ -251,7 +263,7 @@ def main():
             "peak": VALU_PEAK_TOPS,
             "unit": "Tops/s",
             "frac": achieved / VALU_PEAK_TOPS,
-            "traffic": None,
+            "traffic": traffic_bytes(),
             "ops_per_launch": ops,
         },
     }
