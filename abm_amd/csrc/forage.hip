@@ -1,28 +1,586 @@
-// forage.hip -- foraging (Agent.update) step.  Placeholder until the kernel lands.
+// forage.hip -- collective-foraging agent step (Agent.update for every agent
+// in id order) in float64, plus the Agent.projection_field probe.
+//
+// One workgroup per replica; the replica's agents live in LDS and are
+// updated in place in id order (Group.update, abm/simulation/sims.py:861).
+// Per focal agent k:
+//   1. lanes over the other agents: cue class (agent.py:396-419) and the
+//      projection record of each obstacle (agent.py:501-556): closed angle with
+//      the strict 0 < a' < pi rule, the focal radius for both centres, int()
+//      truncation of t -+ proj_size/2, strict FOV test;
+//   2. visual exclusion (agent.py:421-445): records ranked by (distance,
+//      obstacle-list position); each record then walks the nearer ones in rank
+//      order and applies the three interval rules sequentially;
+//   3. raster of the social records with Python slice semantics
+//      (agent.py:569-590) into a per-ray byte field; flip and FOV mask
+//      (agent.py:593-595) when counting;
+//   4. one lane: I_priv, w/u integrators, mode table, random walk from the
+//      host-supplied uniforms of numpy's global stream, relocation force,
+//      Euler step and walls (agent.py:168-283, supcalc.py:38-47, 81-92).
+// Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "../../include/abm_amd.h"
 #include "forage_batch.hpp"
+#include "vf_exact.hpp"
 
-namespace abm { int set_error(int code, const char* fmt, ...); }
+namespace abm {
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+
+namespace {
+
+// Obstacle classes, in the order the reference lists them to projection_field.
+constexpr int kSocial = 0;      // exploiting agents of other patches (the cues)
+constexpr int kNonExpl = 1;     // in-range agents that are not exploiting (self included)
+constexpr int kSamePatch = 2;   // exploiting agents of the focal's own patch
+constexpr int kNone = -1;
+
+struct PairFG {
+    int skipped, visible, t, start, end;
+    double dist, vis;
+};
+
+// One obstacle of Agent.projection_field (agent.py:501-556).
+ABM_HD PairFG pair_forage_exact(const Focal& f, double ox, double oy, int R, double step,
+                                double fov0, double fov1) {
+#pragma clang fp contract(off)
+    PairFG p;
+    p.skipped = 0; p.visible = 0; p.t = 0; p.start = 0; p.end = 0; p.dist = 0.0; p.vis = 0.0;
+    if (ox == f.px && oy == f.py) { p.skipped = 1; return p; }
+    const double v2ex = ox + f.radius;          // the focal radius for the obstacle too
+    const double v2ey = oy + f.radius;
+    const double v2x = v2ex - f.cx, v2y = v2ey - f.cy;
+    const double n2 = norm2(v2x, v2y);
+    const double u2x = v2x / n2, u2y = v2y / n2;
+    double d = dot2(f.u1x, f.u1y, u2x, u2y);
+    if (d < -1.0) d = -1.0;
+    else if (d > 1.0) d = 1.0;
+    double a = acos(d);
+    const double cross = f.u1x * u2y - f.u1y * u2x;
+    if (cross < 0.0) a = -a;
+    a = remainder_2pi(a);
+    const double ca = (0.0 < a && a < kPi) ? -a : kTwoPi - a;   // strict (agent.py:520)
+    p.dist = n2;
+    p.vis = 2.0 * atan(f.radius / (1.0 * n2));
+    p.t = linspace_argmin(ca, R, step);
+    p.visible = (fov0 < ca && ca < fov1) ? 1 : 0;
+    const double ps = (p.vis / kTwoPi) * (double)R;
+    p.start = (int)trunc((double)p.t - ps / 2.0);
+    p.end = (int)trunc((double)p.t + ps / 2.0);
+    return p;
+}
+
+// Indices of range(R)[a:b] (Python slice semantics), as a half-open range.
+ABM_HD void py_slice(int a, int b, int R, int* lo, int* hi) {
+    if (a < 0) { a += R; if (a < 0) a = 0; }
+    if (a > R) a = R;
+    if (b < 0) { b += R; if (b < 0) b = 0; }
+    if (b > R) b = R;
+    *lo = a; *hi = b > a ? b : a;
+}
+
+struct Rec {
+    int valid;        // visible record (obstacle passed the FOV test)
+    int social;
+    int order;        // position in the reference's obstacle list
+    int start, end;   // int(t -+ proj_size/2)
+    int sx, ex;       // after exclusion
+    double dist;
+    double vis;
+};
+
+// Paint one record with Python slice semantics: wrap pieces get `wrap_code`,
+// the main slice `main_code` (agent.py:577-590).
+__device__ inline void paint_record(uint8_t* V, const Rec& rc, int R, uint8_t wrap_code,
+                                    uint8_t main_code) {
+    int s = rc.sx, e = rc.ex, lo, hi;
+    if (s < 0) {
+        py_slice(R + s, R, R, &lo, &hi);
+        for (int r = lo; r < hi; ++r) V[r] = wrap_code;
+        s = 0;
+    }
+    if (e >= R) {
+        py_slice(0, e - R, R, &lo, &hi);
+        for (int r = lo; r < hi; ++r) V[r] = wrap_code;
+        e = R - 1;
+    }
+    py_slice(s, e, R, &lo, &hi);
+    for (int r = lo; r < hi; ++r) V[r] = main_code;
+}
+
+// Per-replica constants in device memory.
+struct FGRep {
+    int32_t agent_offset, n_agents, n_rays, words, visual_exclusion, patchwise, tau, pad_;
+    int64_t field_offset;
+    double fov0, fov1, width, height, pad, vision_range, step;
+    double T_w, Eps_w, g_w, B_w, w_max, T_u, Eps_u, g_u, B_u, u_max, S_wu, S_uw, F_N, F_R;
+    double exp_vel_max, exp_theta_min, exp_theta_max, reloc_theta_max, exp_stop_ratio;
+};
+
+__device__ inline int get_mode(int overriding, double w, double T_w) {
+    // agent.py:659-669
+    if (overriding >= 0) return overriding;
+    return w > T_w ? ABM_MODE_RELOCATE : ABM_MODE_EXPLORE;
+}
+
+struct FGSmem {
+    double *x, *y, *th, *v, *w, *u, *cr, *crb, *nov, *rad;
+    int *mode, *over, *env, *pid, *cls;
+    Rec* rec;
+    int* sorted;
+    uint8_t* V;
+    int* cnt;       // [0] total, [1] left half
+    double* lastd;  // keep_distance: distance of the last processed obstacle
+};
+
+__device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
+    FGSmem s;
+    char* p = reinterpret_cast<char*>(base);
+    double** dptrs[] = {&s.x, &s.y, &s.th, &s.v, &s.w, &s.u, &s.cr, &s.crb, &s.nov, &s.rad};
+    for (double** d : dptrs) { *d = reinterpret_cast<double*>(p); p += sizeof(double) * maxN; }
+    s.lastd = reinterpret_cast<double*>(p); p += sizeof(double) * 2;
+    s.rec = reinterpret_cast<Rec*>(p); p += sizeof(Rec) * maxN;
+    int** iptrs[] = {&s.mode, &s.over, &s.env, &s.pid, &s.cls};
+    for (int** q : iptrs) { *q = reinterpret_cast<int*>(p); p += sizeof(int) * maxN; }
+    s.sorted = reinterpret_cast<int*>(p); p += sizeof(int) * maxN;
+    s.cnt = reinterpret_cast<int*>(p); p += sizeof(int) * 4;
+    s.V = reinterpret_cast<uint8_t*>(p);
+    return s;
+}
+
+size_t fg_lds_bytes(int maxN, int maxR) {
+    return sizeof(double) * ((size_t)10 * maxN + 2) + sizeof(Rec) * (size_t)maxN +
+           sizeof(int) * ((size_t)6 * maxN + 4) + (size_t)maxR + 16;
+}
+
+// Records of one focal agent against the obstacle set marked in S.cls
+// (kNone = not an obstacle), visual exclusion, and the raster into S.V.
+// Every thread of the block calls it; it ends synchronised.
+__device__ void project_focal(const FGSmem& S, int N, int R, double step, double fov0, double fov1,
+                              bool exclusion, bool with_nonsocial, const Focal& f,
+                              bool keep_distance = false) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int r = tid; r < R; r += nthr) S.V[r] = 0;
+    for (int j = tid; j < N; j += nthr) {
+        Rec rc;
+        rc.valid = 0; rc.social = 0; rc.order = 0; rc.start = rc.end = rc.sx = rc.ex = 0;
+        rc.dist = 0.0;
+        rc.vis = 0.0;
+        const int c = S.cls[j];
+        if (c == kSocial || (with_nonsocial && (c == kNonExpl || c == kSamePatch))) {
+            const PairFG p = pair_forage_exact(f, S.x[j], S.y[j], R, step, fov0, fov1);
+            if (!p.skipped && p.visible) {
+                rc.valid = 1;
+                rc.social = c == kSocial;
+                rc.order = c * N + j;
+                rc.start = rc.sx = p.start;
+                rc.end = rc.ex = p.end;
+                rc.dist = p.dist;
+                rc.vis = p.vis;
+            }
+        }
+        S.rec[j] = rc;
+    }
+    __syncthreads();
+    if (exclusion) {
+        // rank = position in the stable ascending sort by distance
+        for (int j = tid; j < N; j += nthr) {
+            const Rec rf = S.rec[j];
+            if (!rf.valid) continue;
+            int rank = 0;
+            for (int i = 0; i < N; ++i) {
+                const Rec ri = S.rec[i];
+                if (ri.valid && (ri.dist < rf.dist || (ri.dist == rf.dist && ri.order < rf.order)))
+                    ++rank;
+            }
+            S.sorted[rank] = j;
+        }
+        __syncthreads();
+        for (int j = tid; j < N; j += nthr) {
+            Rec rf = S.rec[j];
+            if (!rf.valid) continue;
+            // walk the nearer records in rank order; rank 0 keeps its interval
+            for (int q = 0; q < N; ++q) {
+                const int i = S.sorted[q];
+                if (i == j) break;
+                const Rec ri = S.rec[i];
+                if (ri.dist < rf.dist) {
+                    if (rf.sx <= ri.start && ri.start <= rf.ex) rf.ex = ri.start;
+                    if (rf.sx <= ri.end && ri.end <= rf.ex) rf.sx = ri.end;
+                    if (ri.start <= rf.sx && ri.end >= rf.ex) { rf.sx = 0; rf.ex = 0; }
+                }
+            }
+            S.rec[j].sx = rf.sx;
+            S.rec[j].ex = rf.ex;
+        }
+        __syncthreads();
+    }
+    if (!keep_distance) {
+        // every record paints 1, so the order of the records does not matter
+        for (int j = tid; j < N; j += nthr) {
+            const Rec rc = S.rec[j];
+            if (!rc.valid || !rc.social) continue;
+            paint_record(S.V, rc, R, 1, 1);
+        }
+    } else if (tid == 0) {
+        // keep_distance_info: wrap pieces paint 1 and main slices the distance
+        // value (code 2); later records overwrite, in descending vis_angle
+        // order with ties in obstacle order (agent.py:567, 569-590)
+        int last_vis_idx = -1;
+        for (int n = 0; n < N; ++n) {
+            int best = -1;
+            for (int j = 0; j < N; ++j) {
+                const Rec rc = S.rec[j];
+                if (!rc.valid || !rc.social) continue;
+                if (last_vis_idx >= 0) {
+                    const Rec lr = S.rec[last_vis_idx];
+                    const bool after = rc.vis < lr.vis || (rc.vis == lr.vis && rc.order > lr.order);
+                    if (!after) continue;
+                }
+                if (best < 0) { best = j; continue; }
+                const Rec br = S.rec[best];
+                if (rc.vis > br.vis || (rc.vis == br.vis && rc.order < br.order)) best = j;
+            }
+            if (best < 0) break;
+            paint_record(S.V, S.rec[best], R, 1, 2);
+            last_vis_idx = best;
+        }
+    }
+    __syncthreads();
+}
+
+// soc_v_field[k] = V[R-1-k] unless linspace_k is outside [fov0, fov1].
+__device__ inline int soc_bit(const uint8_t* V, int k, int R, double step, double fov0, double fov1) {
+    const double ph = linspace_at(k, R, step);
+    if (ph < fov0 || ph > fov1) return 0;
+    return V[R - 1 - k];
+}
+
+__global__ void __launch_bounds__(256)
+forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
+                   const double* __restrict__ uniforms, const int64_t* __restrict__ u_off,
+                   const int32_t* __restrict__ u_cnt, int32_t* __restrict__ u_used,
+                   uint32_t* __restrict__ bits_out, int maxN, int maxR) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) double smem[];
+    const FGRep P = reps[blockIdx.x];
+    const int N = P.n_agents, R = P.n_rays, a0 = P.agent_offset;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    FGSmem S = fg_carve(smem, maxN, maxR);
+    for (int i = tid; i < N; i += nthr) {
+        const int g = a0 + i;
+        S.x[i] = st.x[g]; S.y[i] = st.y[g]; S.th[i] = st.theta[g]; S.v[i] = st.v[g];
+        S.w[i] = st.w[g]; S.u[i] = st.u[g]; S.cr[i] = st.collected_r[g];
+        S.crb[i] = st.collected_r_before[g];
+        double m = st.novelty[(int64_t)g * P.tau];
+        for (int q = 1; q < P.tau; ++q) m = fmax(m, st.novelty[(int64_t)g * P.tau + q]);
+        S.nov[i] = m;
+        S.rad[i] = (double)st.radius[g];
+        S.mode[i] = st.mode[g]; S.over[i] = st.overriding_mode[g];
+        S.env[i] = st.env_status[g]; S.pid[i] = st.exploited_patch_id[g];
+    }
+    const double* ubuf = uniforms + u_off[blockIdx.x];
+    const int ucount = u_cnt[blockIdx.x];
+    int used = 0;   // meaningful in thread 0
+    const int nl = R / 2;
+    __syncthreads();
+
+    for (int k = 0; k < N; ++k) {
+        // ---- cue classes (agent.py:396-413)
+        const double ckx = S.x[k] + S.rad[k], cky = S.y[k] + S.rad[k];
+        const int pid_k = S.pid[k];
+        for (int j = tid; j < N; j += nthr) {
+            const double dx = (S.x[j] + S.rad[j]) - ckx, dy = (S.y[j] + S.rad[j]) - cky;
+            const bool in_range = norm2(dx, dy) <= P.vision_range;
+            const bool expl = in_range && j != k &&
+                              get_mode(S.over[j], S.w[j], P.T_w) == ABM_MODE_EXPLOIT;
+            int c = kNone;
+            if (expl) {
+                if (P.patchwise && S.pid[j] == pid_k) c = kSamePatch;
+                else if (S.pid[j] != -1) c = kSocial;
+            } else if (in_range) {
+                c = kNonExpl;
+            }
+            S.cls[j] = c;
+        }
+        __syncthreads();
+        const Focal f = make_focal(S.x[k], S.y[k], S.th[k], S.rad[k]);
+        const bool excl = P.visual_exclusion != 0;
+        project_focal(S, N, R, P.step, P.fov0, P.fov1, excl, excl, f);
+
+        // ---- the returned field: counts for mean() and the left/right halves
+        if (tid == 0) { S.cnt[0] = 0; S.cnt[1] = 0; }
+        __syncthreads();
+        int tot = 0, left = 0;
+        for (int r = tid; r < R; r += nthr) {
+            const int b = soc_bit(S.V, r, R, P.step, P.fov0, P.fov1);
+            tot += b;
+            if (r < nl) left += b;
+        }
+        if (tot) atomicAdd(&S.cnt[0], tot);
+        if (left) atomicAdd(&S.cnt[1], left);
+        if (bits_out) {
+            const int W = P.words;
+            uint32_t* dst = bits_out + P.field_offset + (int64_t)k * W;
+            for (int wd = tid; wd < W; wd += nthr) {
+                uint32_t m = 0;
+                for (int b = 0; b < 32; ++b) {
+                    const int r = wd * 32 + b;
+                    if (r < R && soc_bit(S.V, r, R, P.step, P.fov0, P.fov1)) m |= 1u << b;
+                }
+                dst[wd] = m;
+            }
+        }
+        __syncthreads();
+
+        // ---- decision and motion (agent.py:168-283)
+        if (tid == 0) {
+            const double count = (double)S.cnt[0], cleft = (double)S.cnt[1];
+            double w = S.w[k], u = S.u[k], vel0 = S.v[k];
+            const double collected = S.cr[k] - S.crb[k];
+            const double ipriv = P.F_N * S.nov[k] + P.F_R * collected;
+            const double w_p = w > P.T_w ? w : 0.0;
+            const double u_p = u > P.T_u ? u : 0.0;
+            const double mean = count / (double)R;
+            const double dw = P.Eps_w * mean - P.g_w * (w - P.B_w) - u_p * P.S_uw;
+            const double du = P.Eps_u * ipriv - P.g_u * (u - P.B_u) - w_p * P.S_wu;
+            w = w + dw;
+            u = u + du;
+            if (w > P.w_max) w = P.w_max;
+            if (w < -P.w_max) w = -P.w_max;
+            if (u > P.u_max) u = P.u_max;
+            if (u < -P.u_max) u = -P.u_max;
+            int over = S.over[k], mode = S.mode[k];
+            double dvel = 0.0, dth = 0.0;
+            const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
+            auto walk = [&]() {
+                const double uu = used < ucount ? ubuf[used] : NAN;
+                ++used;
+                dvel = P.exp_vel_max;
+                dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
+                over = -1; mode = ABM_MODE_EXPLORE;
+            };
+            auto reloc = [&]() {
+                const double lm = cleft / (double)nl;
+                const double rm = (count - cleft) / (double)(R - nl);
+                dvel = P.exp_vel_max - vel0;
+                dth = (lm - rm) * P.reloc_theta_max;
+                over = -1; mode = ABM_MODE_RELOCATE;
+            };
+            auto exploit = [&]() {
+                over = ABM_MODE_EXPLOIT; mode = ABM_MODE_EXPLOIT;
+                dvel = -vel0 * P.exp_stop_ratio;
+                dth = 0.0;
+            };
+            if (over != ABM_MODE_COLLIDE) {   // get_mode() == "collide" only via the override
+                if (!tw && !tu) walk();
+                else if (tw && tu) { if (S.env[k] == 1) exploit(); else reloc(); }
+                else if (tw && !tu) reloc();
+                else { if (S.env[k] == 1) exploit(); else walk(); }
+            }
+            Pose p{S.x[k], S.y[k], S.th[k], vel0};
+            p.theta = prove_orientation(p.theta + dth);
+            p.v = p.v + dvel;
+            // prove_velocity (agent.py:612-620): only in explore mode
+            if (get_mode(over, w, P.T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = P.exp_vel_max;
+            p.x = p.x + p.v * cos(p.theta);
+            p.y = p.y - p.v * sin(p.theta);
+            p = reflect_walls(p, S.rad[k], P.pad, P.width, P.height);
+            S.x[k] = p.x; S.y[k] = p.y; S.th[k] = p.theta; S.v[k] = p.v;
+            S.w[k] = w; S.u[k] = u; S.over[k] = over; S.mode[k] = mode;
+            S.crb[k] = S.cr[k];
+            st.I_priv[a0 + k] = ipriv;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += nthr) {
+        const int g = a0 + i;
+        st.x[g] = S.x[i]; st.y[g] = S.y[i]; st.theta[g] = S.th[i]; st.v[g] = S.v[i];
+        st.w[g] = S.w[i]; st.u[g] = S.u[i]; st.collected_r_before[g] = S.crb[i];
+        st.mode[g] = S.mode[i]; st.overriding_mode[g] = S.over[i];
+    }
+    if (tid == 0 && u_used) u_used[blockIdx.x] = used;
+}
+
+// Agent.projection_field of one focal agent: obstacles [0, n_social) are the
+// social cues, [n_social, n) the non-social occluders (when n_nonsocial >= 0).
+__global__ void __launch_bounds__(256)
+forage_projection_kernel(int R, double step, double fov0, double fov1, int exclusion,
+                         double vision_range, int keep_distance, double fx, double fy,
+                         double fth, double fr, const double* __restrict__ ox,
+                         const double* __restrict__ oy, int n_social, int n_nonsocial,
+                         double* __restrict__ out, int maxN) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) double smem[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = n_social + (n_nonsocial > 0 ? n_nonsocial : 0);
+    FGSmem S = fg_carve(smem, maxN, R);
+    for (int i = tid; i < N; i += nthr) {
+        S.x[i] = ox[i]; S.y[i] = oy[i];
+        S.cls[i] = i < n_social ? kSocial : kNonExpl;
+    }
+    if (tid == 0) S.lastd[0] = 0.0;
+    __syncthreads();
+    const Focal f = make_focal(fx, fy, fth, fr);
+    // the distance left in the reference's loop variable: last non-coincident obstacle
+    if (tid == 0 && keep_distance) {
+        for (int i = N - 1; i >= 0; --i) {
+            const PairFG p = pair_forage_exact(f, S.x[i], S.y[i], R, step, fov0, fov1);
+            if (!p.skipped) { S.lastd[0] = p.dist; break; }
+        }
+    }
+    project_focal(S, N, R, step, fov0, fov1, exclusion != 0, n_nonsocial >= 0, f, keep_distance != 0);
+    const double val = 1.0 - S.lastd[0] / vision_range;
+    for (int r = tid; r < R; r += nthr) {
+        const int b = soc_bit(S.V, r, R, step, fov0, fov1);
+        out[r] = b == 0 ? 0.0 : (b == 2 ? val : 1.0);
+    }
+}
+
+}  // namespace
+}  // namespace abm
+
 using namespace abm;
 
 extern "C" {
-int abm_forage_batch_create(const abm_forage_replica*, int32_t, abm_forage_batch** out) {
-    if (out) *out = nullptr;
-    return set_error(ABM_ERR_UNSUPPORTED, "foraging step not built yet");
+
+int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forage_batch** out) {
+    if (!out) return set_error(ABM_ERR_INVALID, "abm_forage_batch_create: out is NULL");
+    *out = nullptr;
+    if (!reps || n <= 0) return set_error(ABM_ERR_INVALID, "abm_forage_batch_create: no replicas");
+    std::vector<FGRep> host(n);
+    int64_t agents = 0, words = 0;
+    int max_n = 0, max_r = 0;
+    for (int b = 0; b < n; ++b) {
+        const abm_forage_replica& r = reps[b];
+        if (r.n_agents < 1 || r.n_agents > 4096)
+            return set_error(ABM_ERR_INVALID, "forage replica %d: n_agents=%d out of range [1, 4096]", b, r.n_agents);
+        if (r.n_rays < 2 || r.n_rays > (1 << 16))
+            return set_error(ABM_ERR_INVALID, "forage replica %d: n_rays=%d out of range", b, r.n_rays);
+        if (r.tau < 1 || r.tau > 4096)
+            return set_error(ABM_ERR_INVALID, "forage replica %d: tau=%d out of range", b, r.tau);
+        FGRep& d = host[b];
+        std::memset(&d, 0, sizeof(d));
+        d.agent_offset = (int32_t)agents; d.n_agents = r.n_agents; d.n_rays = r.n_rays;
+        d.words = (r.n_rays + 31) / 32; d.visual_exclusion = r.visual_exclusion ? 1 : 0;
+        d.patchwise = r.patchwise_exclusion ? 1 : 0; d.tau = r.tau; d.field_offset = words;
+        d.fov0 = r.fov0; d.fov1 = r.fov1; d.width = r.width; d.height = r.height;
+        d.pad = r.window_pad; d.vision_range = r.vision_range; d.step = linspace_step(r.n_rays);
+        d.T_w = r.T_w; d.Eps_w = r.Eps_w; d.g_w = r.g_w; d.B_w = r.B_w; d.w_max = r.w_max;
+        d.T_u = r.T_u; d.Eps_u = r.Eps_u; d.g_u = r.g_u; d.B_u = r.B_u; d.u_max = r.u_max;
+        d.S_wu = r.S_wu; d.S_uw = r.S_uw; d.F_N = r.F_N; d.F_R = r.F_R;
+        d.exp_vel_max = r.exp_vel_max; d.exp_theta_min = r.exp_theta_min;
+        d.exp_theta_max = r.exp_theta_max; d.reloc_theta_max = r.reloc_theta_max;
+        d.exp_stop_ratio = r.exp_stop_ratio;
+        agents += r.n_agents;
+        words += (int64_t)r.n_agents * d.words;
+        max_n = std::max(max_n, r.n_agents);
+        max_r = std::max(max_r, r.n_rays);
+    }
+    abm_forage_batch* bt = new abm_forage_batch();
+    bt->n_replicas = n; bt->n_agents_total = (int32_t)agents; bt->field_words = words;
+    bt->max_agents = max_n; bt->max_rays = max_r;
+    int rc = check_hip(hipGetDevice(&bt->device), "hipGetDevice");
+    if (!rc) rc = check_hip(hipMalloc(&bt->reps_dev, sizeof(FGRep) * n), "hipMalloc(forage replicas)");
+    if (!rc) rc = check_hip(hipMemcpy(bt->reps_dev, host.data(), sizeof(FGRep) * n, hipMemcpyHostToDevice),
+                           "upload forage replicas");
+    if (rc) { abm_forage_batch_destroy(bt); return rc; }
+    bt->agent_offset_host = new int32_t[n + 1];
+    bt->field_offset_host = new int64_t[n + 1];
+    for (int b = 0; b < n; ++b) {
+        bt->agent_offset_host[b] = host[b].agent_offset;
+        bt->field_offset_host[b] = host[b].field_offset;
+    }
+    bt->agent_offset_host[n] = (int32_t)agents;
+    bt->field_offset_host[n] = words;
+    *out = bt;
+    return ABM_OK;
 }
-int abm_forage_batch_destroy(abm_forage_batch*) { return ABM_OK; }
-int abm_forage_batch_get_info(const abm_forage_batch*, abm_batch_info*) {
-    return set_error(ABM_ERR_UNSUPPORTED, "foraging step not built yet");
+
+int abm_forage_batch_destroy(abm_forage_batch* b) {
+    if (!b) return ABM_OK;
+    if (b->reps_dev) (void)hipFree(b->reps_dev);
+    delete[] b->agent_offset_host;
+    delete[] b->field_offset_host;
+    delete b;
+    return ABM_OK;
 }
-int abm_forage_batch_get_offsets(const abm_forage_batch*, int32_t*, int64_t*) {
-    return set_error(ABM_ERR_UNSUPPORTED, "foraging step not built yet");
+
+int abm_forage_batch_get_info(const abm_forage_batch* b, abm_batch_info* out) {
+    if (!b || !out) return set_error(ABM_ERR_INVALID, "abm_forage_batch_get_info: NULL argument");
+    out->n_replicas = b->n_replicas; out->n_agents_total = b->n_agents_total;
+    out->field_words = b->field_words; out->max_agents = b->max_agents; out->max_rays = b->max_rays;
+    return ABM_OK;
 }
-int abm_forage_step_f64(abm_forage_batch*, const abm_forage_state*, const double*, const int64_t*,
-                        const int32_t*, int32_t*, uint32_t*, void*) {
-    return set_error(ABM_ERR_UNSUPPORTED, "foraging step not built yet");
+
+int abm_forage_batch_get_offsets(const abm_forage_batch* b, int32_t* agent_offset, int64_t* field_offset) {
+    if (!b) return set_error(ABM_ERR_INVALID, "abm_forage_batch_get_offsets: NULL batch");
+    if (agent_offset) std::copy(b->agent_offset_host, b->agent_offset_host + b->n_replicas + 1, agent_offset);
+    if (field_offset) std::copy(b->field_offset_host, b->field_offset_host + b->n_replicas + 1, field_offset);
+    return ABM_OK;
 }
-int abm_forage_projection(int32_t, double, double, int32_t, double, int32_t, double, double, double,
-                          double, const double*, const double*, int32_t, int32_t, double*, void*) {
-    return set_error(ABM_ERR_UNSUPPORTED, "foraging step not built yet");
+
+int abm_forage_step_f64(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
+                        const int64_t* uniform_offset, const int32_t* uniform_count,
+                        int32_t* uniforms_used, uint32_t* field_bits, void* stream) {
+    if (!b || !st) return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: NULL batch or state");
+    const void* req[] = {st->x, st->y, st->theta, st->v, st->w, st->u, st->I_priv, st->collected_r,
+                         st->collected_r_before, st->novelty, st->mode, st->overriding_mode,
+                         st->env_status, st->exploited_patch_id, st->radius};
+    for (const void* p : req)
+        if (!p) return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: every state array is required");
+    if (!uniforms || !uniform_offset || !uniform_count)
+        return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: the uniform buffer is required");
+    const int nthr = b->max_agents > 128 ? 256 : (b->max_agents > 64 ? 128 : 64);
+    const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
+    int dev_max = 0;
+    if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                 b->device), "hipDeviceGetAttribute"))
+        return rc;
+    if (lds > (size_t)dev_max)
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_step_f64: %d agents x %d rays need %zu B of LDS",
+                         b->max_agents, b->max_rays, lds);
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
+                       (const FGRep*)b->reps_dev, *st, uniforms, uniform_offset, uniform_count,
+                       uniforms_used, field_bits, b->max_agents, b->max_rays);
+    return check_hip(hipGetLastError(), "forage_step_kernel launch");
 }
+
+int abm_forage_projection(int32_t R, double fov0, double fov1, int32_t visual_exclusion,
+                          double vision_range, int32_t keep_distance, double fx, double fy,
+                          double ftheta, double fradius, const double* ox, const double* oy,
+                          int32_t n_social, int32_t n_nonsocial, double* out_field, void* stream) {
+    if (R < 2 || R > (1 << 16)) return set_error(ABM_ERR_INVALID, "abm_forage_projection: n_rays=%d", R);
+    if (n_social < 0) return set_error(ABM_ERR_INVALID, "abm_forage_projection: n_social < 0");
+    const int n = n_social + (n_nonsocial > 0 ? n_nonsocial : 0);
+    if (n > 4096) return set_error(ABM_ERR_INVALID, "abm_forage_projection: more than 4096 obstacles");
+    if (!out_field || (n > 0 && (!ox || !oy)))
+        return set_error(ABM_ERR_INVALID, "abm_forage_projection: NULL array");
+    const int maxN = n > 0 ? n : 1;
+    const size_t lds = fg_lds_bytes(maxN, R);
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_projection_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    hipLaunchKernelGGL(forage_projection_kernel, dim3(1), dim3(256), lds, (hipStream_t)stream, R,
+                       linspace_step(R), fov0, fov1, visual_exclusion, vision_range, keep_distance,
+                       fx, fy, ftheta, fradius, ox, oy, n_social, n_nonsocial, out_field, maxN);
+    return check_hip(hipGetLastError(), "forage_projection_kernel launch");
 }
+
+}  // extern "C"
