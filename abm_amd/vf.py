@@ -151,6 +151,20 @@ class VFSwarm:
                                   device=self.device)
         self.diag = torch.zeros(6 * n, dtype=dt, device=self.device)
 
+    def load(self, x=None, y=None, theta=None, v=None, radius=None, alp0=None, bet0=None, v0=None):
+        """Overwrite state arrays in place from host values (None keeps an
+        array; per-agent overrides are replaced when given)."""
+        for name, val in (("x", x), ("y", y), ("theta", theta), ("v", v), ("radius", radius)):
+            if val is not None:
+                t = getattr(self, name)
+                t.copy_(torch.as_tensor(np.asarray(val).reshape(t.shape), dtype=t.dtype))
+        n = self.x.numel()
+        dt = self.x.dtype
+        for name, val in (("alp0", alp0), ("bet0", bet0), ("v0", v0)):
+            if val is not None:
+                setattr(self, name, torch.as_tensor(np.asarray(val, dtype=np.float64).reshape(n),
+                                                    dtype=dt).to(self.device))
+
     # -------------------------------------------------------------- stepping
     def _state(self):
         S = L.VFStateF64 if self.precision == "f64" else L.VFStateF32

@@ -390,3 +390,134 @@ def agents_arrays(agents):
         exploited_patch_id=np.array([a.exploited_patch_id for a in agents], dtype=np.int32),
         radius=np.array([a.radius for a in agents], dtype=np.int32),
         soc_v_field=np.stack([np.asarray(a.soc_v_field) for a in agents]))
+
+
+# ------------------------------------------------ resources and host loop
+class Patch:
+    """Rescource attributes the loop uses (abm/environment/rescource.py:15-56)."""
+
+    def __init__(self, id, radius, position, units, quality):
+        self.id = id
+        self.radius = radius
+        self.resc_units = units
+        self.resc_left = units
+        self.position = np.array(position, dtype=np.float64)
+        self.center = (self.position[0] + self.radius, self.position[1] + self.radius)
+        self.unit_per_timestep = quality
+
+    def deplete(self, units):
+        # rescource.py:118-133
+        if units > self.unit_per_timestep:
+            units = self.unit_per_timestep
+        if self.resc_left >= units:
+            self.resc_left -= units
+            dep = units
+        else:
+            dep = self.resc_left
+            self.resc_left = 0
+        return dep, not (self.resc_left > 0)
+
+
+def _collide_circle(c1, r1, c2, r2):
+    # pygame.sprite.collide_circle on integer rect centres (patch centres are
+    # integers: randint positions + integer radius)
+    dx, dy = c1[0] - c2[0], c1[1] - c2[1]
+    return dx * dx + dy * dy <= (r1 + r2) ** 2
+
+
+class ForageWorld:
+    """Simulation state for the headless foraging loop (sims.py:702-870) with
+    agent collisions off (AGENT_AGENT_COLLISION=0)."""
+
+    def __init__(self, agents, W, H, pad, n_resc, resc_radius, min_units, max_units,
+                 min_quality, max_quality, regenerate=True, teleport_exploit=False,
+                 allow_border_overlap=False, agent_consumption=1):
+        self.agents = agents
+        self.W, self.H, self.pad = W, H, pad
+        self.resc_radius = resc_radius
+        self.min_units, self.max_units = min_units, max_units
+        if max_quality < 0:
+            max_quality = min_quality            # sims.py:174-177
+        if max_units < 0:
+            self.max_units = min_units + 1
+        self.min_quality, self.max_quality = min_quality, max_quality
+        self.regenerate = regenerate
+        self.teleport_exploit = teleport_exploit
+        self.allow_border_overlap = allow_border_overlap
+        self.consumption = agent_consumption
+        self.patches: List[Patch] = []
+        for _ in range(n_resc):
+            self.add_patch()
+
+    def add_patch(self, force_id=None):
+        """add_new_resource_patch (sims.py:328-368)."""
+        if force_id is None:
+            pid = max([p.id for p in self.patches]) if self.patches else 0
+        else:
+            pid = force_id
+        retries = 0
+        while True:
+            if retries > 10000:
+                raise Exception("Reached timeout while trying to create resources without overlap!")
+            r = self.resc_radius
+            if self.allow_border_overlap:
+                x = np.random.randint(self.pad - r, self.W + self.pad - r)
+                y = np.random.randint(self.pad - r, self.H + self.pad - r)
+            else:
+                x = np.random.randint(self.pad, self.W + self.pad - 2 * r)
+                y = np.random.randint(self.pad, self.H + self.pad - 2 * r)
+            units = np.random.randint(self.min_units, self.max_units)
+            quality = np.random.uniform(self.min_quality, self.max_quality)
+            new = Patch(pid + 1 if force_id is None else pid, r, (x, y), units, quality)
+            ok = not any(_collide_circle((int(p.center[0]), int(p.center[1])), p.radius,
+                                         (int(new.center[0]), int(new.center[1])), new.radius)
+                         for p in self.patches)
+            retries += 1
+            if ok:
+                break
+        self.patches.append(new)
+        return new.id
+
+    def step(self):
+        """One simulation step: agent-resource interaction (sims.py:789-858),
+        then the agents' update (sims.py:861)."""
+        on_patch = []
+        # groupcollide + refine_ar_overlap_group (sims.py:45-56, 790-799): the
+        # membership of every patch is fixed before the loop
+        groups = [(p, [a for a in self.agents if distance(p, a) < p.radius]) for p in self.patches]
+        for p, here in groups:
+            if not here:
+                continue
+            destroy = False
+            for a in here:
+                x1, y1 = a.position + a.radius                 # bias_agent_towards_res_center
+                x2, y2 = p.center
+                cl = (np.arctan2(y2 - y1, x2 - x1) + a.orientation) % (np.pi * 2)
+                a.orientation += (cl - np.pi) * 0.02
+                if destroy:
+                    notify_agent(a, -1)
+                else:
+                    if (a.get_mode() in ["pool", "relocate"] and a.pool_success) or a.pooling_time == 0:
+                        notify_agent(a, 1, p.id)
+                        if self.teleport_exploit:
+                            a.position = p.position + p.radius - a.radius
+                    if a.get_mode() == "exploit":
+                        dep, destroy = p.deplete(self.consumption)
+                        a.collected_r_before = a.collected_r
+                        a.collected_r += dep
+                        if destroy:
+                            for b in here:
+                                notify_agent(b, -1)
+                on_patch.append(a)
+            if destroy:
+                rid = p.id
+                self.patches.remove(p)
+                if self.regenerate:
+                    self.add_patch(force_id=rid)
+        for a in self.agents:
+            if a not in on_patch:
+                if (a.get_mode() in ["pool", "relocate"] and a.pool_success) or a.pooling_time == 0:
+                    notify_agent(a, -1)
+                elif a.get_mode() == "exploit":
+                    notify_agent(a, -1)
+        group_update(self.agents)
