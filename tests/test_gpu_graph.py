@@ -1,0 +1,63 @@
+"""Graph capture of the step and the launch-shape autotuner.
+
+A captured step is replayed several times, with unrelated eager work and
+synchronisations between replays; every replay must equal an eager twin
+stepped alongside (same kernels, same inputs, so bit-identical)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def sweep(n_side=8, N=50, R=512, precision="f32", seed0=10000):
+    from abm_amd.vf import VFConfig, VFSwarm
+    a = np.linspace(0.1, 3.0, n_side)
+    cfgs, xs, ys, ts = [], [], [], []
+    for i in range(n_side):
+        for j in range(n_side):
+            cfgs.append(VFConfig(n_agents=N, n_rays=R, width=707, height=707, boundary="infinite",
+                                 ALP0=float(a[i]), BET0=float(a[j])))
+            rng = np.random.RandomState(seed0 + i * n_side + j)
+            xs.append(rng.uniform(30, 737, N)); ys.append(rng.uniform(30, 737, N))
+            ts.append(rng.uniform(0, 2 * np.pi, N))
+    f = lambda v: np.concatenate(v).astype(np.float32).astype(np.float64)
+    return VFSwarm(cfgs, f(xs), f(ys), f(ts), 0.5, radius=5, precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_graph_replays_match_eager(gpu, precision):
+    g_sw, e_sw = sweep(precision=precision), sweep(precision=precision)
+    g_sw.step()                                   # first launch outside capture
+    e_sw.step()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        g_sw.step()
+    # capture does not execute: replay 1 is the second step
+    junk = torch.randn(1 << 20, device="cuda")
+    for rep in range(5):
+        graph.replay()
+        e_sw.step()
+        junk = torch.sort(junk)[0] * 1.0001      # unrelated eager work between replays
+        torch.cuda.synchronize()
+        for name in ("x", "y", "theta", "v"):
+            assert torch.equal(getattr(g_sw, name), getattr(e_sw, name)), (rep, name)
+
+
+def test_autotune_and_launch_shapes_agree(gpu):
+    sw = sweep(n_side=6)
+    w, ms = sw.autotune(trial_steps=2)
+    assert w in (1, 2, 4) and ms > 0
+    outs = []
+    for nw in (1, 2, 4):
+        t = sweep(n_side=6)
+        t.set_launch(nw)
+        t.step(fields=True)
+        outs.append((t.fields.clone(), t.state_numpy()))
+    for f, st in outs[1:]:
+        assert torch.equal(f, outs[0][0])          # identical rasters
+        for a, b in zip(st, outs[0][1]):
+            assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)) < 1e-6
