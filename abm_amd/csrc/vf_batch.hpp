@@ -58,6 +58,7 @@ struct abm_vf_batch {
     abm::RayTab32* tab32 = nullptr;
     unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks
     int32_t* order_dev = nullptr;   // replica launch order (largest work first)
+    float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
     // host copies
     abm::VFRep* reps_host = nullptr;
     int32_t* agent_offset_host = nullptr;

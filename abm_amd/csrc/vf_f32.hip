@@ -22,6 +22,8 @@
 //              float32 per-lane partials, float64 cross-lane tree.
 //   motion  -- float32 VSWRM update, Euler step, boundary (vf_agent.py:188-204,
 //              263-326; agent.py:347-395, 605-610).
+#include <cmath>
+
 #include "vf_batch.hpp"
 #include "vf_exact.hpp"
 
@@ -41,6 +43,9 @@ constexpr float kThreeHalfPiF = 4.71238898038468985769f;
 // half-width band is 4e-6 relative + 1e-6 (its float32 error is < 1e-6
 // relative).  Pairs inside a band are re-decided in float64.
 constexpr float kAngleGuardF = 4e-6f;
+
+// Replicas with more agents than this take the wide (synchronous) kernel.
+constexpr int kWideThreshold = 2048;
 
 // Wave sum: float32 DPP adds inside each 16-lane row (quad xor 1, xor 2,
 // half-row mirror, row mirror), then float64 across rows (row_bcast15 into
@@ -485,6 +490,291 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide variant for single large flocks, synchronous order only: one wave per
+// focal agent, four waves per workgroup.  The waves of a workgroup share the
+// ray table and 256-agent tiles of the (pre-step) poses in LDS.  A cheap
+// float32 distance pass first drops every pair beyond the exact visibility
+// cut-off -- the raster of a pair is empty when floor(proj_size / 2) = 0,
+// i.e. when d > size * cot(2 pi / R) -- with a relative guard of 2e-4, and
+// compacts the others; only those take the float32 decision.  New poses go
+// to a scratch buffer and are copied back by a second kernel.
+constexpr int kWideWaves = 4;
+constexpr int kWideTile = 256;
+
+struct WideSmem {
+    RayTab32* tab;      // transposed for 64 lanes
+    float4* tile;       // (x, y, radius, -) of the current object tile
+    uint32_t* F;        // kWideWaves fields of maxW words
+    int32_t* cand;      // kWideWaves lists of tile-local candidates
+};
+
+template <int RPL>
+__global__ void __launch_bounds__(64 * kWideWaves)
+vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
+                        abm_vf_state_f32 st, float4* __restrict__ out_pose,
+                        uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
+                        unsigned long long* __restrict__ counters, float cull_c) {
+    extern __shared__ __align__(16) double smem_d[];
+    const RepConst C = rep_const(rp);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int k = blockIdx.x * kWideWaves + wave;
+    const bool active = k < C.N;
+    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const int maxW = (RPL * 64 + 31) >> 5;
+    WideSmem S;
+    {
+        char* p = reinterpret_cast<char*>(smem_d);
+        S.tab = reinterpret_cast<RayTab32*>(p); p += sizeof(RayTab32) * RPL * 64;
+        S.tile = reinterpret_cast<float4*>(p); p += sizeof(float4) * kWideTile;
+        S.cand = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * kWideTile * kWideWaves;
+        S.F = reinterpret_cast<uint32_t*>(p);
+    }
+    uint32_t* F = S.F + wave * maxW;
+    int32_t* cand = S.cand + wave * kWideTile;
+    const int tab_off = rp->table_offset;
+    for (int e = tid; e < RPL * 64; e += 64 * kWideWaves) {
+        const int r = (e % 64) * RPL + e / 64;
+        RayTab32 z;
+        z.wcos = z.wsin = z.cos = z.sin = 0.f;
+        S.tab[e] = r < C.R ? tab32[tab_off + r] : z;
+    }
+    for (int w = lane; w < maxW; w += 64) F[w] = 0u;
+    const int g = C.a0 + (active ? k : 0);
+    const float fx = st.x[g], fy = st.y[g], fth = st.theta[g], fr = (float)st.radius[g];
+    float sth, cth;
+    sincosf(fth, &sth, &cth);
+    const float u1x = cth, u1y = -sth;
+    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+
+    for (int t0 = 0; t0 < C.N; t0 += kWideTile) {
+        __syncthreads();
+        if (t0 + tid < C.N) {
+            const int gj = C.a0 + t0 + tid;
+            S.tile[tid] = make_float4(st.x[gj], st.y[gj], (float)st.radius[gj], 0.f);
+        }
+        __syncthreads();
+        // cheap pass: candidates inside the cut-off (or inside a torus guard band)
+        int ncand = 0;
+        const int tn = C.N - t0 < kWideTile ? C.N - t0 : kWideTile;
+        for (int sub = 0; sub < kWideTile; sub += 64) {
+            const int jt = sub + lane;
+            bool c = false, culled = false;
+            if (active && jt < tn && t0 + jt != k) {
+                const float4 o = S.tile[jt];
+                if (!(o.x == fx && o.y == fy)) {
+                    if (count) ++n_pairs;
+                    float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
+                    bool near_wrap = false;
+                    if (C.torus) {
+                        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+                        near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
+                        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+                        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+                    }
+                    const float lim = o.z * cull_c;
+                    culled = !near_wrap && fmaf(dy, dy, dx * dx) > lim * lim;
+                    c = !culled || verify;
+                }
+            }
+            const uint64_t m = __ballot(c);
+            if (c) {
+                const int pos = ncand + __popcll(m & ((1ull << lane) - 1ull));
+                cand[pos] = jt | (culled ? (1 << 30) : 0);
+            }
+            ncand += __popcll(m);
+        }
+        __syncthreads();
+        // float32 decisions of the candidates (float64 when inside a guard band)
+        for (int c0 = 0; c0 < ncand; c0 += 64) {
+            const int ci = c0 + lane;
+            if (ci >= ncand) continue;
+            const int e = cand[ci];
+            const int jt = e & ((1 << 30) - 1);
+            const bool culled = (e >> 30) & 1;
+            const float4 o = S.tile[jt];
+            int start, end, drawn;
+            bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn);
+            if (!ok || verify) {
+                int es, ee, ed;
+                exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                if (verify) {
+                    // a culled pair must have an empty raster; a fast decision must match
+                    const bool empty = !ed || es == ee;
+                    if ((culled && !empty) ||
+                        (!culled && ok && (es != start || ee != end || ed != drawn)))
+                        ++n_disagree;
+                }
+                if (!ok) {
+                    if (count) ++n_recheck;
+                    start = es; end = ee; drawn = ed;
+                }
+            }
+            if (culled) continue;
+            if (drawn) raster(F, start, end, C.R);
+        }
+    }
+    __syncthreads();
+    if (!active) {
+        if (count) {
+            if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        }
+        return;
+    }
+    // sums over this wave's field (lanes own RPL consecutive rays)
+    const int rs = lane * RPL;
+    const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
+    const bool case1 = (F[0] & 1u) == 1u && field_bit(F, C.R - 1) == 0u;
+    float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
+    {
+        const int w0 = rs >> 5;
+        const uint32_t lo = w0 < C.W ? F[w0] : 0u;
+        const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? F[w0 + 1] : 0u;
+        const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
+        const uint32_t m = low_mask(rn);
+        const uint32_t cur = (uint32_t)v64 & m;
+        uint32_t spike = 0u;
+        if (rn > 0) {
+            if (case1) {
+                const uint32_t prev = field_bit(F, rs == 0 ? C.R - 1 : rs - 1);
+                spike = (cur ^ ((cur << 1) | prev)) & m;
+            } else {
+                const int nx = rs + rn;
+                const uint32_t next = field_bit(F, nx >= C.R ? 0 : nx);
+                spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const RayTab32 T = S.tab[i * 64 + lane];
+            const bool vb = (cur >> i) & 1u;
+            const bool sb = (spike >> i) & 1u;
+            bc += vb ? T.wcos : 0.f;
+            bs += vb ? T.wsin : 0.f;
+            ec += sb ? T.cos : 0.f;
+            es += sb ? T.sin : 0.f;
+        }
+    }
+    const float alp0 = st.alp0 ? st.alp0[g] : (float)rp->alp0;
+    const float bet0 = st.bet0 ? st.bet0[g] : (float)rp->bet0;
+    const float v0 = st.v0 ? st.v0[g] : (float)rp->v0;
+    const double Bc = wave_total(bc), Bs = wave_total(bs), Ec = wave_total(ec), Es = wave_total(es);
+    const double Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+    const double Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+    if (bits_out) {
+        uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
+        for (int w = lane; w < C.W; w += 64) dst[w] = F[w];
+    }
+    float vel = st.v[g];
+    float dv = 0.f, dpsi = 0.f;
+    if (C.phi_ok) {
+        dv = C.gam * (v0 - vel) + (float)Sv;
+        dpsi = (float)Sp;
+    }
+    float theta = dpsi;
+    if (C.limit && fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
+    float o = wrap_2pi(fth + theta);
+    vel = vel + dv;
+    if (C.limit && fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
+    float so, co;
+    sincosf(o, &so, &co);
+    float nx = fx + vel * co;
+    float ny = fy - vel * so;
+    const float xc = nx + fr, yc = ny + fr;
+    const float lo_b = C.pad, hx_b = C.pad + C.fW, hy_b = C.pad + C.fH;
+    if (C.torus) {
+        if (xc < lo_b) nx = hx_b - fr;
+        else if (xc > hx_b) nx = lo_b + fr;
+        if (yc < lo_b) ny = hy_b - fr;
+        else if (yc > hy_b) ny = lo_b + fr;
+    } else {
+        if (xc < lo_b) {
+            nx = lo_b - fr;
+            if (kHalfPiF <= o && o < kPiF) o -= kHalfPiF;
+            else if (kPiF <= o && o <= kThreeHalfPiF) o += kHalfPiF;
+            o = wrap_2pi(o);
+        }
+        if (xc > hx_b) {
+            nx = hx_b - fr - 1.f;
+            if (kThreeHalfPiF <= o && o < kTwoPiF) o -= kHalfPiF;
+            else if (0.f <= o && o <= kHalfPiF) o += kHalfPiF;
+            o = wrap_2pi(o);
+        }
+        if (yc < lo_b) {
+            ny = lo_b - fr;
+            if (kHalfPiF <= o && o <= kPiF) o += kHalfPiF;
+            else if (0.f <= o && o < kHalfPiF) o -= kHalfPiF;
+            o = wrap_2pi(o);
+        }
+        if (yc > hy_b) {
+            ny = hy_b - fr - 1.f;
+            if (kThreeHalfPiF <= o && o <= kTwoPiF) o += kHalfPiF;
+            else if (kPiF <= o && o < kThreeHalfPiF) o -= kHalfPiF;
+            o = wrap_2pi(o);
+        }
+    }
+    if (lane == 0) {
+        out_pose[g] = make_float4(nx, ny, o, vel);
+        if (want_diag) {
+            float* dd = diag + 6 * (int64_t)g;
+            dd[0] = dv; dd[1] = dpsi;
+            dd[2] = C.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
+            dd[3] = C.phi_ok ? (float)((double)(alp0 * C.alp1) * Ec) : 0.f;
+            dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
+            dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
+        }
+    }
+    if (count) {
+        if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+    }
+}
+
+__global__ void copy_back_kernel(const float4* __restrict__ out_pose, abm_vf_state_f32 st, int a0, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = out_pose[a0 + i];
+    st.x[a0 + i] = p.x;
+    st.y[a0 + i] = p.y;
+    st.theta[a0 + i] = p.z;
+    st.v[a0 + i] = p.w;
+}
+
+size_t lds_bytes_wide(int rpl) {
+    const int maxW = (rpl * 64 + 31) >> 5;
+    return sizeof(RayTab32) * (size_t)rpl * 64 + sizeof(float4) * kWideTile +
+           sizeof(int32_t) * kWideTile * kWideWaves + sizeof(uint32_t) * (size_t)maxW * kWideWaves;
+}
+
+template <int RPL>
+int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+                uint32_t flags, hipStream_t s) {
+    const size_t lds = lds_bytes_wide(RPL);
+    auto kern = vf_step_f32_wide_kernel<RPL>;
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    for (int rep = 0; rep < b->n_replicas; ++rep) {
+        const VFRep& h = b->reps_host[rep];
+        // visibility cut-off: floor(proj_size/2) >= 1 needs d <= size * cot(2 pi / R)
+        const float cull_c = (float)((1.0 / std::tan(kTwoPi / h.n_rays)) * (1.0 + 2e-4));
+        const int blocks = (h.n_agents + kWideWaves - 1) / kWideWaves;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWideWaves), lds, s, b->reps_dev + rep,
+                           b->tab32, *st, b->wide_out, bits, diag, flags, b->counters, cull_c);
+        if (int rc = check_hip(hipGetLastError(), "vf_step_f32_wide_kernel launch")) return rc;
+        hipLaunchKernelGGL(copy_back_kernel, dim3((h.n_agents + 255) / 256), dim3(256), 0, s,
+                           (const float4*)b->wide_out, *st, h.agent_offset, h.n_agents);
+        if (int rc = check_hip(hipGetLastError(), "copy_back_kernel launch")) return rc;
+    }
+    return ABM_OK;
+}
+
 size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
     const int maxW = (maxR + 31) >> 5;
     size_t b = sizeof(RayTab32) * (size_t)rpl * 64 * nw + sizeof(float4) * (size_t)maxN * (sync_order ? 3 : 2) +
@@ -534,6 +824,21 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
     if (b->max_rays > 2048)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: n_rays up to 2048 (got %d); use abm_vf_step_f64", b->max_rays);
+    const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
+    if (b->max_agents > kWideThreshold) {
+        if (!sync_order)
+            return set_error(ABM_ERR_UNSUPPORTED,
+                             "abm_vf_step_f32: %d agents in one replica do not fit one CU's LDS; the "
+                             "reference (sequential) order is not available at this size, use "
+                             "ABM_ORDER_SYNCHRONOUS", b->max_agents);
+        if (!b->wide_out)
+            return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: batch has no wide-step buffer");
+        const int need = (b->max_rays + 63) / 64;
+        if (need <= 4) return launch_wide<4>(b, st, bits, diag, flags, s);
+        if (need <= 8) return launch_wide<8>(b, st, bits, diag, flags, s);
+        if (need <= 16) return launch_wide<16>(b, st, bits, diag, flags, s);
+        return launch_wide<32>(b, st, bits, diag, flags, s);
+    }
     int nw = b->waves_per_group;
     if (nw == 0) nw = b->max_agents > 256 ? 4 : (b->max_agents > 96 ? 2 : 1);
     const int need = (b->max_rays + 64 * nw - 1) / (64 * nw);   // rays per lane

@@ -1,0 +1,57 @@
+"""Single large flocks (more agents than one CU's LDS holds) take the wide
+synchronous kernel.  Checks: every float32 decision and every pair dropped
+by the visibility cut-off agree with float64 (verify counters), and sampled
+agents match the oracle on the pre-step poses (synchronous order)."""
+import numpy as np
+import pytest
+
+from oracle import vf_oracle as vo
+
+pytestmark = pytest.mark.gpu
+PI = np.pi
+
+
+def flock(N, R, W, seed, r=2):
+    from abm_amd.vf import VFConfig, VFSwarm
+    rng = np.random.RandomState(seed)
+    f = lambda a: a.astype(np.float32).astype(np.float64)
+    x, y = f(rng.uniform(30, 30 + W, N)), f(rng.uniform(30, 30 + W, N))
+    th, v = f(rng.uniform(0, 2 * PI, N)), f(rng.uniform(0, 1.5, N))
+    cfg = VFConfig(n_agents=N, n_rays=R, width=W, height=W, boundary="infinite", ALP0=1.0, BET0=0.5)
+    return cfg, (x, y, th, v), VFSwarm([cfg], x, y, th, v, radius=r, precision="f32")
+
+
+def test_reference_order_is_refused_for_a_large_flock(gpu):
+    from abm_amd import AbmUnsupported
+    cfg, st, sw = flock(2100, 512, 1000, 1)
+    with pytest.raises(AbmUnsupported):
+        sw.step(order="sequential")
+
+
+def test_large_flock_sampled_agents_match_oracle(gpu):
+    cfg, (x, y, th, v), sw = flock(3000, 2048, 2400, 7)
+    sw.step(order="synchronous", fields=True, diag=True, verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert disagree == 0 and pairs == 3000 * 2999
+    bits = sw.fields_numpy(0)
+    nx, ny, nth, nv = sw.state_numpy()
+    p = vo.VFParams(ALP0=1.0, BET0=0.5, BOUNDARY="infinite")
+    for k in np.random.RandomState(0).choice(3000, 12, replace=False):
+        ox, oy, oth, ov, obits, _ = vo.agent_view_step(x, y, th, v, 2, k, 2048, cfg.fov, 2400, 2400,
+                                                       30, p)
+        assert np.array_equal(bits[k], obits), k
+        d = [abs((nx[k] - ox + 1200) % 2400 - 1200) / max(abs(ox), 1),
+             abs((ny[k] - oy + 1200) % 2400 - 1200) / max(abs(oy), 1),
+             abs((nth[k] - oth + PI) % (2 * PI) - PI) / max(abs(oth), 1),
+             abs(nv[k] - ov) / max(abs(ov), 1)]
+        assert max(d) <= 1e-5, (k, d)
+
+
+def test_c5_flock_verify(gpu):
+    # N=8192, radius 2, 4000x4000 torus, R=2048: every decision checked
+    cfg, st, sw = flock(8192, 2048, 4000, 11)
+    for _ in range(3):
+        sw.step(order="synchronous", verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert disagree == 0
+    assert pairs == 3 * 8192 * 8191
