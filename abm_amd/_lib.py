@@ -96,6 +96,8 @@ _D = C.c_double
 _SIGS = {
     "abm_last_error": (C.c_char_p, []),
     "abm_build_info": (C.c_char_p, []),
+    "abm_mt19937_peek": (C.c_int, [_P, _I32, _P, _P]),
+    "abm_mt19937_skip": (C.c_int, [_P, _I32, _P]),
     "abm_vf_batch_create": (C.c_int, [C.POINTER(VFReplica), _I32, C.POINTER(_P)]),
     "abm_vf_batch_destroy": (C.c_int, [_P]),
     "abm_vf_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),

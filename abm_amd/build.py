@@ -34,6 +34,7 @@ UNITS = {
     "vf_f64.hip": ["-ffp-contract=off"],
     "vf_f32.hip": ["-ffp-contract=fast"],
     "forage.hip": ["-ffp-contract=off"],
+    "mt19937.hip": [],
 }
 
 

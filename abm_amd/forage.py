@@ -132,6 +132,12 @@ class ForageSwarm:
         self.u_off = torch.as_tensor(self.agent_offset[:-1].astype(np.int64), device=self.device)
         self.u_cnt = torch.as_tensor(np.diff(self.agent_offset).astype(np.int32), device=self.device)
         self.u_used = torch.zeros(nb, dtype=torch.int32, device=self.device)
+        # uniforms drawn ahead: N per replica, staged in pinned host memory
+        self._counts = np.diff(self.agent_offset).astype(np.int64)
+        self._u_host = torch.zeros(n, dtype=torch.float64, pin_memory=True)
+        self.u_dev = torch.zeros(n, dtype=torch.float64, device=self.device)
+        self._addr, self._addr_of = None, []
+        self._check_rng_layout()
 
     def close(self):
         if getattr(self, "handle", None):
@@ -160,37 +166,64 @@ class ForageSwarm:
         return slice(int(self.agent_offset[b]), int(self.agent_offset[b + 1]))
 
     # ---------------------------------------------------------------- step
+    def _state_addrs(self):
+        """Addresses of the replicas' MT19937 states (numpy's mt19937_state,
+        the layout of abm_mt19937_state)."""
+        gens = self.rngs if self.rngs is not None else [np.random.mtrand._rand]
+        if self.rngs is None or self._addr is None or any(
+                g is not c for g, c in zip(gens, self._addr_of)):
+            self._addr_of = list(gens)
+            self._addr = np.array([g._bit_generator.ctypes.state_address for g in gens],
+                                  dtype=np.uint64)
+        return self._addr
+
     def _draw_ahead(self):
         """Uniform doubles each replica would draw next, without consuming them."""
-        bufs = []
-        for b, cfg in enumerate(self.configs):
-            src = self.rngs[b] if self.rngs is not None else None
-            st = src.get_state() if src is not None else np.random.get_state()
-            tmp = np.random.RandomState()
-            tmp.set_state(st)
-            bufs.append(tmp.random_sample(cfg.n_agents))
-        return np.concatenate(bufs)
+        addrs = self._state_addrs()
+        L.check(self.lib.abm_mt19937_peek(addrs.ctypes.data_as(C.c_void_p), len(addrs),
+                                          self._counts.ctypes.data_as(C.c_void_p),
+                                          self._u_host.data_ptr()))
+        return self._u_host
 
     def _advance(self, used):
-        for b, k in enumerate(used):
-            if k:
-                if self.rngs is not None:
-                    self.rngs[b].random_sample(int(k))
-                else:
-                    np.random.random_sample(int(k))
+        used = np.ascontiguousarray(used, dtype=np.int64)
+        addrs = self._state_addrs()
+        L.check(self.lib.abm_mt19937_skip(addrs.ctypes.data_as(C.c_void_p), len(addrs),
+                                          used.ctypes.data_as(C.c_void_p)))
+
+    def _check_rng_layout(self):
+        """numpy's generator state must be the published mt19937_state struct:
+        compare a peek with draws from a copy of the first generator."""
+        g = self.rngs[0] if self.rngs is not None else np.random.mtrand._rand
+        tmp = np.random.RandomState()
+        tmp.set_state(g.get_state())
+        want = tmp.random_sample(4)
+        got = np.zeros(4)
+        addr = np.array([g._bit_generator.ctypes.state_address], dtype=np.uint64)
+        cnt = np.array([4], dtype=np.int64)
+        L.check(self.lib.abm_mt19937_peek(addr.ctypes.data_as(C.c_void_p), 1,
+                                          cnt.ctypes.data_as(C.c_void_p),
+                                          got.ctypes.data_as(C.c_void_p)))
+        if not np.array_equal(got, want):
+            raise L.AbmError(L.ABM_ERR_UNSUPPORTED, "numpy's MT19937 state layout is not the "
+                             "expected mt19937_state struct")
 
     def step(self, fields=False, stream=None):
         """One Agent.update for every agent of every replica; returns the
         number of uniforms each replica consumed."""
-        u = torch.as_tensor(self._draw_ahead(), device=self.device)
-        st = L.ForageState(**{k: self.t[k].data_ptr() for k in
-                              _F64 + ("novelty",) + _I32})
-        fb = L.ptr(self.fields) if fields else None
         with torch.cuda.device(self.device):
-            L.check(self.lib.abm_forage_step_f64(self.handle, C.byref(st), L.ptr(u),
-                                                 L.ptr(self.u_off), L.ptr(self.u_cnt),
-                                                 L.ptr(self.u_used), fb, L.stream_ptr(stream)))
-        used = self.u_used.cpu().numpy()
+            s = stream if stream is not None else torch.cuda.current_stream()
+            with torch.cuda.stream(s):
+                # the pinned staging buffer is free again: the previous step
+                # ended with a synchronising read of u_used on this stream
+                self.u_dev.copy_(self._draw_ahead(), non_blocking=True)
+                st = L.ForageState(**{k: self.t[k].data_ptr() for k in
+                                      _F64 + ("novelty",) + _I32})
+                fb = L.ptr(self.fields) if fields else None
+                L.check(self.lib.abm_forage_step_f64(self.handle, C.byref(st), L.ptr(self.u_dev),
+                                                     L.ptr(self.u_off), L.ptr(self.u_cnt),
+                                                     L.ptr(self.u_used), fb, L.stream_ptr(s)))
+                used = self.u_used.cpu().numpy()
         self._advance(used)
         return used
 

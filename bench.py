@@ -147,6 +147,107 @@ def cpu_baseline(seconds=15.0):
                        f"agent-steps/s")}
 
 
+def _line(args, workload, value, ms, dtype, config, **extra):
+    out = {"metric": METRIC, "value": value, "unit": "agent-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+           "config": dict(workload=workload, **config)}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def other_workload(args):
+    """Measurements of the other configurations (not the headline line)."""
+    import torch
+    from abm_amd.vf import VFConfig, VFSwarm
+    from abm_amd.forage import ForageConfig, ForageSwarm
+    f32 = lambda a: np.asarray(a).astype(np.float32).astype(np.float64)
+    if args.workload == "single":
+        Ns, Rs, Ws = 100, 1024, 1000.0
+        rng = np.random.RandomState(42)
+        x, y = f32(rng.uniform(PAD, PAD + Ws, Ns)), f32(rng.uniform(PAD, PAD + Ws, Ns))
+        th = f32(rng.uniform(0, 2 * np.pi, Ns))
+        cfg = VFConfig(n_agents=Ns, n_rays=Rs, width=Ws, height=Ws, boundary="infinite", ALP0=1.0,
+                       BET0=0.5)
+        res = {}
+        for prec in ("f32", "f64"):
+            sw = VFSwarm([cfg], x, y, th, 0.0, radius=5, precision=prec)
+            res[prec] = time_steps(sw, args.steps, args.warmup, not args.no_graph)[0]
+        ops = ops_per_agent_step(Ns, Rs) * Ns
+        _line(args, f"single VSWRM replica N={Ns}, R={Rs}, 1000x1000 torus, reference order", Ns / (res["f32"] * 1e-3),
+              res["f32"], "f32", {"replicas": 1, "agents_per_replica": Ns, "rays": Rs},
+              plain_f64={"agent_steps_per_s": Ns / (res["f64"] * 1e-3), "ms_per_step": res["f64"]},
+              speedup_vs_plain_f64=res["f64"] / res["f32"],
+              roofline={"bound": "valu", "achieved": ops / (res["f32"] * 1e-3) / 1e12,
+                        "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                        "frac": ops / (res["f32"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, "traffic": None,
+                        "note": "one replica occupies one workgroup: latency-bound by design"})
+    elif args.workload == "large":
+        Nl, Rl, Wl = 8192, 2048, 4000.0
+        rng = np.random.RandomState(7)
+        x, y = f32(rng.uniform(PAD, PAD + Wl, Nl)), f32(rng.uniform(PAD, PAD + Wl, Nl))
+        th = f32(rng.uniform(0, 2 * np.pi, Nl))
+        cfg = VFConfig(n_agents=Nl, n_rays=Rl, width=Wl, height=Wl, boundary="infinite", ALP0=1.0,
+                       BET0=0.5)
+        sw = VFSwarm([cfg], x, y, th, 0.5, radius=2, precision="f32")
+
+        class Sync:
+            step = staticmethod(lambda: sw.step(order="synchronous"))
+        ms = time_steps(Sync, args.steps, args.warmup, not args.no_graph)[0]
+        ops = ops_per_agent_step(Nl, Rl) * Nl
+        _line(args, f"one flock N={Nl}, R={Rl}, radius 2, 4000x4000 torus, synchronous order (wide kernel)",
+              Nl / (ms * 1e-3), ms, "f32", {"replicas": 1, "agents_per_replica": Nl, "rays": Rl,
+                                             "order": "synchronous"},
+              roofline={"bound": "valu", "achieved": ops / (ms * 1e-3) / 1e12, "peak": VALU_PEAK_TOPS,
+                        "unit": "Tops/s", "frac": ops / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                        "traffic": None,
+                        "note": "nominal all-pairs ops; pairs beyond the exact visibility cut-off are dropped"},
+              pair_ray_evals_per_s=Nl * (Nl - 1) * Rl / (ms * 1e-3))
+    else:   # mixed ragged batch: even replicas forage, odd replicas flock
+        rng = np.random.RandomState(20000)
+        vf_cfgs, fg_cfgs, vx, fx_ = [], [], [], []
+        for b in range(2000):
+            n = int(round(np.exp(rng.uniform(np.log(8), np.log(512)))))
+            r = int(rng.choice([256, 512, 1024]))
+            side = 100.0 * np.sqrt(n)
+            srng = np.random.RandomState(20000 + b)
+            pos = (f32(srng.uniform(PAD, PAD + side, n)), f32(srng.uniform(PAD, PAD + side, n)),
+                   f32(srng.uniform(0, 2 * np.pi, n)))
+            if b % 2:
+                vf_cfgs.append(VFConfig(n_agents=n, n_rays=r, width=side, height=side,
+                                        boundary="infinite", ALP0=1.0, BET0=0.5))
+                vx.append(pos)
+            else:
+                fg_cfgs.append(ForageConfig(n_agents=n, n_rays=r, width=side, height=side,
+                                            vision_range=2000.0))
+                fx_.append(pos)
+        cat = lambda L, i: np.concatenate([p[i] for p in L])
+        vsw = VFSwarm(vf_cfgs, cat(vx, 0), cat(vx, 1), cat(vx, 2), 0.5, radius=5, precision="f32")
+        fsw = ForageSwarm(fg_cfgs, cat(fx_, 0), cat(fx_, 1), cat(fx_, 2), radius=10,
+                          rngs=[np.random.RandomState(30000 + i) for i in range(len(fg_cfgs))])
+        vsw.autotune(trial_steps=2)
+        ms_vf = time_steps(vsw, args.steps, args.warmup, not args.no_graph)[0]
+        # the foraging step reads back the uniforms it used each step: eager launches
+        for _ in range(min(args.warmup, 3)):
+            fsw.step()
+        torch.cuda.synchronize()
+        n_fs = max(3, min(args.steps, 20))
+        t0 = time.perf_counter()
+        for _ in range(n_fs):
+            fsw.step()
+        torch.cuda.synchronize()
+        ms_fg = (time.perf_counter() - t0) * 1e3 / n_fs
+        n_vf = int(vsw.batch.info.n_agents_total)
+        n_fg = int(fsw.info.n_agents_total)
+        ms = ms_vf + ms_fg
+        _line(args, "2000 ragged replicas (N log-uniform 8..512, R in {256,512,1024}, side 100*sqrt(N)): "
+                    "odd replicas flock (fused float32), even replicas forage (float64)",
+              (n_vf + n_fg) / (ms * 1e-3), ms, "f32+f64",
+              {"replicas": 2000, "agents_flocking": n_vf, "agents_foraging": n_fg},
+              flocking={"agent_steps_per_s": n_vf / (ms_vf * 1e-3), "ms_per_step": ms_vf},
+              foraging={"agent_steps_per_s": n_fg / (ms_fg * 1e-3), "ms_per_step_wall": ms_fg})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +257,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "single", "mixed", "large"],
+                    help="sweep: 4096 replicas x N=50 x R=512 (default, the headline); single: "
+                         "one replica N=100, R=1024, 1000x1000 torus; mixed: 2000 ragged replicas, "
+                         "half foraging, half flocking; large: one flock N=8192, R=2048, "
+                         "4000x4000 torus, synchronous order")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -173,6 +279,9 @@ def main():
         print(f"bench.py runs on one GPU: the replicas are independent and the workload "
               f"is one GPU's share; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "sweep":
+        return other_workload(args)
 
     # the CPU baseline runs first, in worker processes started before this
     # process touches the GPU

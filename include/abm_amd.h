@@ -249,6 +249,24 @@ int abm_forage_projection(int32_t n_rays, double fov0, double fov1, int32_t visu
                           const double* ox, const double* oy, int32_t n_social,
                           int32_t n_nonsocial, double* out_field, void* stream);
 
+/* ------------------------------------------------ host random numbers */
+/* numpy's legacy MT19937 state (numpy/random/src/mt19937/mt19937.h,
+ * mt19937_state): the address a RandomState's bit generator reports as
+ * ctypes.state_address points at exactly this struct. */
+typedef struct abm_mt19937_state {
+    uint32_t key[624];
+    int32_t pos;
+} abm_mt19937_state;
+
+/* Host functions (no GPU).  Replace the per-replica draws of
+ * np.random.uniform in the random walk (abm/agent/supcalc.py:38-47):
+ * abm_mt19937_peek writes, for each state b, the next counts[b] doubles that
+ * random_sample would return, consecutively into out, leaving the states
+ * unchanged; abm_mt19937_skip advances each state b by counts[b] draws. */
+int abm_mt19937_peek(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts,
+                     double* out);
+int abm_mt19937_skip(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts);
+
 /* --------------------------------------------------------------- misc */
 const char* abm_last_error(void);
 /* Library version string and the gfx target it was built for. */

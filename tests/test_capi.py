@@ -69,3 +69,55 @@ def test_bit_packing_roundtrip():
     for R in (8, 31, 32, 33, 512, 1000):
         b = (rng.rand(3, R) < 0.3).astype(np.uint8)
         assert np.array_equal(unpack_bits(pack_bits(b), R), b)
+
+
+def _addrs(gens):
+    return np.array([g._bit_generator.ctypes.state_address for g in gens], dtype=np.uint64)
+
+
+def test_mt19937_peek_and_skip_follow_numpys_legacy_stream():
+    # host-only helpers of the random-walk protocol: peek must equal draws
+    # from a copy of each RandomState, skip must leave each generator where
+    # random_sample(count) leaves it; positions straddle the 624-word twist
+    lib = L.load()
+    gens = [np.random.RandomState(s) for s in range(40)]
+    for i, g in enumerate(gens):
+        g.random_sample((i * 37) % 700)
+    gens[0].get_state()                       # fresh twist pending (pos 624) on some
+    counts = np.array([(i * 53) % 1500 for i in range(40)], dtype=np.int64)
+    addr = _addrs(gens)
+    out = np.zeros(int(counts.sum()))
+    before = [g.get_state() for g in gens]
+    assert lib.abm_mt19937_peek(addr.ctypes.data_as(C.c_void_p), len(gens),
+                                counts.ctypes.data_as(C.c_void_p),
+                                out.ctypes.data_as(C.c_void_p)) == 0
+    want = []
+    for g, c in zip(gens, counts):
+        t = np.random.RandomState()
+        t.set_state(g.get_state())
+        want.append(t.random_sample(int(c)))
+    assert np.array_equal(out, np.concatenate(want))
+    for g, b in zip(gens, before):               # peek changed nothing
+        assert g.get_state()[2] == b[2] and np.array_equal(g.get_state()[1], b[1])
+    assert lib.abm_mt19937_skip(addr.ctypes.data_as(C.c_void_p), len(gens),
+                                counts.ctypes.data_as(C.c_void_p)) == 0
+    for g, b, c in zip(gens, before, counts):
+        t = np.random.RandomState()
+        t.set_state(b)
+        t.random_sample(int(c))
+        assert g.get_state()[2] == t.get_state()[2]
+        assert np.array_equal(g.get_state()[1], t.get_state()[1])
+        assert g.uniform(-0.3, 0.3) == t.uniform(-0.3, 0.3)
+
+
+def test_mt19937_rejects_bad_arguments():
+    lib = L.load()
+    g = np.random.RandomState(1)
+    addr = _addrs([g])
+    neg = np.array([-1], dtype=np.int64)
+    assert lib.abm_mt19937_skip(addr.ctypes.data_as(C.c_void_p), 1,
+                                neg.ctypes.data_as(C.c_void_p)) == L.ABM_ERR_INVALID
+    assert b"negative" in lib.abm_last_error()
+    one = np.array([1], dtype=np.int64)
+    assert lib.abm_mt19937_peek(addr.ctypes.data_as(C.c_void_p), 1,
+                                one.ctypes.data_as(C.c_void_p), None) == L.ABM_ERR_INVALID
