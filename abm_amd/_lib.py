@@ -107,6 +107,7 @@ _SIGS = {
     "abm_vf_f32_fallback_count": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                             C.POINTER(C.c_int64), _P]),
     "abm_vf_set_launch": (C.c_int, [_P, _I32]),
+    "abm_vf_set_grouping": (C.c_int, [_P, _I32]),
     "abm_vf_autotune": (C.c_int, [_P, C.POINTER(VFStateF32), _I32, C.POINTER(_I32),
                                   C.POINTER(C.c_float), _P]),
     "abm_vf_projection_rows": (C.c_int, [_I32, _D, _D, _I32, _D, _D, _D, _D, _D, _D,

@@ -33,6 +33,7 @@ UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "vf_f64.hip": ["-ffp-contract=off"],
     "vf_f32.hip": ["-ffp-contract=fast"],
+    "vf_group.hip": ["-ffp-contract=fast"],
     "forage.hip": ["-ffp-contract=off"],
     "mt19937.hip": [],
 }

@@ -100,6 +100,85 @@ static int probe_tables(int R, ProbeTables* out) {
     return ABM_OK;
 }
 
+// Replicas that may share a wave of the grouped kernel: every constant the
+// kernel reads from the group's first replica must agree.
+static bool group_class_less(const VFRep& a, const VFRep& b) {
+    const double ka[] = {(double)a.n_rays, (double)a.boundary, (double)a.fov_px0, (double)a.fov_px1,
+                         (double)a.phi_ok, (double)a.limit_movement, a.width, a.height, a.pad,
+                         a.gam, a.alp1, a.bet1, a.max_vel, a.max_th, a.step};
+    const double kb[] = {(double)b.n_rays, (double)b.boundary, (double)b.fov_px0, (double)b.fov_px1,
+                         (double)b.phi_ok, (double)b.limit_movement, b.width, b.height, b.pad,
+                         b.gam, b.alp1, b.bet1, b.max_vel, b.max_th, b.step};
+    for (size_t i = 0; i < sizeof(ka) / sizeof(ka[0]); ++i) {
+        if (ka[i] < kb[i]) return true;
+        if (ka[i] > kb[i]) return false;
+    }
+    return false;
+}
+
+// Groups of G replicas (same class, similar N, largest first) and the nibble
+// tables of R for G replicas per wave: lane li of a group owns rays
+// [li * RPL, (li + 1) * RPL); entry [q][v][li] of the first table is the sum of
+// (trapz weight * cos PHI, * sin PHI) over the rays li*RPL + 4q + i whose bit
+// i is set in v, the second table the same sum of (cos PHI, sin PHI).
+static int build_group_plan(abm_vf_batch* bt, const std::vector<VFRep>& host, int G, int R,
+                            const HostTables& t) {
+    GroupPlan& gp = bt->gplan[G == 1 ? 0 : (G == 2 ? 1 : 2)];
+    const int n = (int)host.size();
+    std::vector<int> idx(n);
+    for (int b = 0; b < n; ++b) idx[b] = b;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) {
+        if (group_class_less(host[a], host[c])) return true;
+        if (group_class_less(host[c], host[a])) return false;
+        return host[a].n_agents > host[c].n_agents;
+    });
+    std::vector<std::vector<int>> groups;
+    for (int i = 0; i < n;) {
+        std::vector<int> g{idx[i]};
+        int j = i + 1;
+        while (j < n && (int)g.size() < G && !group_class_less(host[idx[i]], host[idx[j]]) &&
+               !group_class_less(host[idx[j]], host[idx[i]]))
+            g.push_back(idx[j++]);
+        while ((int)g.size() < G) g.push_back(-1);
+        groups.push_back(g);
+        i = j;
+    }
+    std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int>& a, const std::vector<int>& c) {
+        return host[a[0]].n_agents > host[c[0]].n_agents;
+    });
+    std::vector<int32_t> flat;
+    for (auto& g : groups) flat.insert(flat.end(), g.begin(), g.end());
+    const int L = 64 / G, rpl = group_rpl(G, R), nq = rpl / 4;
+    const int tab = 4 * rpl * L;
+    std::vector<float2> nib(2 * (size_t)tab);
+    for (int li = 0; li < L; ++li)
+        for (int q = 0; q < nq; ++q)
+            for (int v = 0; v < 16; ++v) {
+                double bx = 0, by = 0, sx = 0, sy = 0;
+                for (int i = 0; i < 4; ++i) {
+                    const int r = li * rpl + 4 * q + i;
+                    if (!((v >> i) & 1) || r >= R) continue;
+                    const double wl = r > 0 ? t.d[r - 1] : 0.0;
+                    const double wr = r + 1 < R ? t.d[r] : 0.0;
+                    const double w = 0.5 * (wl + wr);
+                    bx += w * t.c[r]; by += w * t.s[r];
+                    sx += t.c[r]; sy += t.s[r];
+                }
+                const size_t e = (size_t)(q * 16 + v) * L + li;
+                nib[e] = make_float2((float)bx, (float)by);
+                nib[tab + e] = make_float2((float)sx, (float)sy);
+            }
+    gp.n_groups = (int32_t)groups.size();
+    gp.tab_entries = tab;
+    int rc = check_hip(hipMalloc(&gp.groups, sizeof(int32_t) * flat.size()), "hipMalloc(groups)");
+    if (!rc) rc = check_hip(hipMalloc(&gp.nib, sizeof(float2) * nib.size()), "hipMalloc(nibble tables)");
+    if (!rc) rc = check_hip(hipMemcpy(gp.groups, flat.data(), sizeof(int32_t) * flat.size(),
+                                      hipMemcpyHostToDevice), "upload groups");
+    if (!rc) rc = check_hip(hipMemcpy(gp.nib, nib.data(), sizeof(float2) * nib.size(),
+                                      hipMemcpyHostToDevice), "upload nibble tables");
+    return rc;
+}
+
 static bool is_finite(double x) { return std::isfinite(x); }
 
 }  // namespace abm
@@ -218,6 +297,14 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
     if (!rc) rc = check_hip(hipMemcpy(bt->tab32, t32.data(), sizeof(RayTab32) * ray_total, hipMemcpyHostToDevice), "upload tab32");
     if (!rc) rc = check_hip(hipMemcpy(bt->order_dev, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "upload order");
     if (!rc) rc = check_hip(hipMemset(bt->counters, 0, sizeof(unsigned long long) * 4), "hipMemset(counters)");
+    // the grouped kernel: one R in the batch, replicas small enough for a wave's LDS share
+    if (!rc && table_off.size() == 1 && max_r <= kGroupMaxRays && max_n <= kGroupMaxAgents) {
+        HostTables t;
+        make_tables(max_r, &t);
+        for (int G : {1, 2, 4})
+            if (!rc) rc = build_group_plan(bt, host, G, max_r, t);
+        if (!rc) bt->group_ok = 1;
+    }
     if (rc) { abm_vf_batch_destroy(bt); return rc; }
     bt->reps_host = new VFRep[n];
     std::copy(host.begin(), host.end(), bt->reps_host);
@@ -241,6 +328,10 @@ int abm_vf_batch_destroy(abm_vf_batch* b) {
     if (b->counters) (void)hipFree(b->counters);
     if (b->order_dev) (void)hipFree(b->order_dev);
     if (b->wide_out) (void)hipFree(b->wide_out);
+    for (GroupPlan& gp : b->gplan) {
+        if (gp.groups) (void)hipFree(gp.groups);
+        if (gp.nib) (void)hipFree(gp.nib);
+    }
     delete[] b->reps_host;
     delete[] b->agent_offset_host;
     delete[] b->field_offset_host;
@@ -308,6 +399,21 @@ int abm_vf_set_launch(abm_vf_batch* b, int32_t waves) {
     if (waves != 0 && waves != 1 && waves != 2 && waves != 4)
         return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2 or 4");
     b->waves_per_group = waves;
+    b->group_g = waves > 0 ? -1 : 0;
+    return ABM_OK;
+}
+
+int abm_vf_set_grouping(abm_vf_batch* b, int32_t replicas_per_wave) {
+    if (!b) return set_error(ABM_ERR_INVALID, "abm_vf_set_grouping: NULL batch");
+    const int g = replicas_per_wave;
+    if (g != -1 && g != 0 && g != 1 && g != 2 && g != 4)
+        return set_error(ABM_ERR_INVALID, "abm_vf_set_grouping: replicas_per_wave must be -1, 0, 1, 2 or 4");
+    if (g > 0 && !b->group_ok)
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_set_grouping: the grouped kernel needs one n_rays "
+                         "(<= %d) in the batch and at most %d agents per replica", kGroupMaxRays,
+                         kGroupMaxAgents);
+    b->group_g = g;
+    if (g != 0) b->waves_per_group = 0;
     return ABM_OK;
 }
 
@@ -319,12 +425,16 @@ int abm_vf_autotune(abm_vf_batch* b, const abm_vf_state_f32* st, int32_t trial_s
     hipEvent_t e0, e1;
     if (int rc = check_hip(hipEventCreate(&e0), "hipEventCreate")) return rc;
     if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) { (void)hipEventDestroy(e0); return rc; }
-    const int cand[3] = {1, 2, 4};
+    // candidates: waves per replica of the per-replica kernel (> 0), or minus the
+    // replicas per wave of the grouped kernel
+    std::vector<int> cand{1, 2, 4};
+    if (b->group_ok) { cand.push_back(-1); cand.push_back(-2); cand.push_back(-4); }
     int best = 1;
     float best_ms = 1e30f;
     int rc = ABM_OK;
-    for (int c = 0; c < 3 && !rc; ++c) {
-        b->waves_per_group = cand[c];
+    for (size_t c = 0; c < cand.size() && !rc; ++c) {
+        if (cand[c] > 0) { b->waves_per_group = cand[c]; b->group_g = -1; }
+        else { b->waves_per_group = 0; b->group_g = -cand[c]; }
         rc = launch_vf_step_f32(b, st, nullptr, nullptr, 0, s);  // warm-up
         if (!rc) rc = check_hip(hipEventRecord(e0, s), "hipEventRecord");
         for (int i = 0; i < trial_steps && !rc; ++i) rc = launch_vf_step_f32(b, st, nullptr, nullptr, 0, s);
@@ -337,7 +447,8 @@ int abm_vf_autotune(abm_vf_batch* b, const abm_vf_state_f32* st, int32_t trial_s
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    b->waves_per_group = best;
+    if (best > 0) { b->waves_per_group = best; b->group_g = -1; }
+    else { b->waves_per_group = 0; b->group_g = -best; }
     if (chosen) *chosen = best;
     if (chosen_ms) *chosen_ms = best_ms / (float)trial_steps;
     return ABM_OK;

@@ -35,6 +35,18 @@ struct RayTab32 {
     float cos, sin;     // cos(PHI_k), sin(PHI_k)
 };
 
+// Replica groups of the grouped fused kernel (vf_group.hip) for one group size G.
+struct GroupPlan {
+    int32_t* groups = nullptr;   // [n_groups][G] replica ids, -1 = empty slot
+    int32_t n_groups = 0;
+    float2* nib = nullptr;       // nibble tables (blob, then spike) in the kernel's LDS layout
+    int32_t tab_entries = 0;     // float2 entries per table
+};
+
+// Largest replica the grouped kernel takes, and the largest R.
+constexpr int kGroupMaxAgents = 256;
+constexpr int kGroupMaxRays = 1024;
+
 }  // namespace abm
 
 // The opaque handle.
@@ -46,7 +58,9 @@ struct abm_vf_batch {
     int32_t max_rays = 0;
     int32_t n_tables = 0;
     int32_t ray_total = 0;          // entries in each ray table
-    int32_t waves_per_group = 0;    // fused kernel launch shape (0 = default)
+    int32_t waves_per_group = 0;    // per-replica fused kernel: waves per replica (0 = default)
+    int32_t group_ok = 0;           // the grouped kernel can run this batch
+    int32_t group_g = 0;            // 0 = automatic, -1 = per-replica kernel, 1/2/4 = replicas per wave
     int device = 0;
     int dev_max_lds = 0;            // device limits, read once at creation
     int n_cu = 0;
@@ -59,6 +73,7 @@ struct abm_vf_batch {
     unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks
     int32_t* order_dev = nullptr;   // replica launch order (largest work first)
     float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
+    abm::GroupPlan gplan[3];        // G = 1, 2, 4
     // host copies
     abm::VFRep* reps_host = nullptr;
     int32_t* agent_offset_host = nullptr;
@@ -71,6 +86,10 @@ int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bi
                        double* diag, uint32_t flags, hipStream_t s);
 int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
                        float* diag, uint32_t flags, hipStream_t s);
+int launch_vf_group_f32(abm_vf_batch* b, int G, const abm_vf_state_f32* st, uint32_t* bits,
+                        float* diag, uint32_t flags, hipStream_t s);
+int group_rpl(int G, int R);
+int selected_group_size(const abm_vf_batch* b);
 int set_error(int code, const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 }  // namespace abm

@@ -24,189 +24,15 @@
 //              263-326; agent.py:347-395, 605-610).
 #include <cmath>
 
-#include "vf_batch.hpp"
-#include "vf_exact.hpp"
+#include "vf_f32_common.hpp"
 
 namespace abm {
 
 namespace {
-
-constexpr float kPiF = 3.14159265358979323846f;
-constexpr float kTwoPiF = 6.28318530717958647692f;
-constexpr float kHalfPiF = 1.57079632679489661923f;
-constexpr float kThreeHalfPiF = 4.71238898038468985769f;
-
-// Guard bands.  The float32 angle (sincos of the heading, centre
-// differences, cross/dot, atan2_f32) is within 1e-6 rad of the exact one and
-// the float64 reference angle within 5e-8 rad, so a 4e-6 rad band around
-// every bin tie and around the +-pi wrap covers both with margin.  The
-// half-width band is 4e-6 relative + 1e-6 (its float32 error is < 1e-6
-// relative).  Pairs inside a band are re-decided in float64.
-constexpr float kAngleGuardF = 4e-6f;
+using namespace f32;
 
 // Replicas with more agents than this take the wide (synchronous) kernel.
 constexpr int kWideThreshold = 2048;
-
-// Wave sum: float32 DPP adds inside each 16-lane row (quad xor 1, xor 2,
-// half-row mirror, row mirror), then float64 across rows (row_bcast15 into
-// rows 1 and 3, row_bcast31 into rows 2 and 3); lane 63 ends with the total,
-// returned as a wave-uniform double.
-template <int CTRL>
-__device__ inline float dpp_f32(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
-}
-template <int CTRL, int ROW_MASK>
-__device__ inline double dpp_f64_masked(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline double wave_total(float x) {
-    x += dpp_f32<0xB1>(x);          // quad_perm [1,0,3,2]
-    x += dpp_f32<0x4E>(x);          // quad_perm [2,3,0,1]
-    x += dpp_f32<0x141>(x);         // row_half_mirror
-    x += dpp_f32<0x140>(x);         // row_mirror: every lane holds its row's sum
-    double v = (double)x;
-    v += dpp_f64_masked<0x142, 0xa>(v);   // row_bcast15 -> rows 1, 3
-    v += dpp_f64_masked<0x143, 0xc>(v);   // row_bcast31 -> rows 2, 3
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ inline float lane_f32(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-__device__ inline uint32_t low_mask(int n) { return n >= 32 ? 0xffffffffu : ((1u << n) - 1u); }
-
-__device__ inline uint32_t field_bit(const uint32_t* F, int r) { return (F[r >> 5] >> (r & 31)) & 1u; }
-
-// OR the rays [a, b) into the bit field.
-__device__ inline void or_range(uint32_t* F, int a, int b) {
-    while (a < b) {
-        const int w = a >> 5;
-        const int lo = a & 31;
-        const int hi = (b - (w << 5)) < 32 ? (b - (w << 5)) : 32;
-        const uint32_t m = low_mask(hi) & ~low_mask(lo);
-        atomicOr(&F[w], m);
-        a = (w + 1) << 5;
-    }
-}
-
-__device__ inline void raster(uint32_t* F, int start, int end, int R) {
-    int p0, p1, q0, q1;
-    vf_pieces(start, end, R, &p0, &p1, &q0, &q1);
-    or_range(F, p0, p1);
-    or_range(F, q0, q1);
-}
-
-__device__ inline float wrap_2pi(float o) {
-    if (o < 0.f) o = kTwoPiF + o;
-    if (o > kTwoPiF) o = o - kTwoPiF;
-    return o;
-}
-
-// atan2 for float32 from a degree-7 minimax polynomial of atan(z)/z in z^2 on
-// [0, 1] (|error| <= 1.5e-7 with fused multiply-adds) and octant reduction;
-// total error below 4e-7 rad.  The guard bands allow for it.
-__device__ inline float atan2_f32(float y, float x) {
-    const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    const float z = mn * __builtin_amdgcn_rcpf(mx);
-    const float s = z * z;
-    float p = -0.0046934f;
-    p = fmaf(p, s, 0.02425289f);
-    p = fmaf(p, s, -0.05948712f);
-    p = fmaf(p, s, 0.09914348f);
-    p = fmaf(p, s, -0.14019503f);
-    p = fmaf(p, s, 0.19969729f);
-    p = fmaf(p, s, -0.3333199f);
-    p = fmaf(p, s, 0.9999999f);
-    p = p * z;
-    if (ay > ax) p = kHalfPiF - p;
-    if (x < 0.f) p = kPiF - p;
-    return y < 0.f ? -p : p;
-}
-
-// Per-replica constants, all float32 except the few integers.
-struct RepConst {
-    int N, R, W, a0, torus, fov_px0, fov_px1, phi_ok, limit;
-    long field_offset;
-    float inv_step;   // (R - 1) / (2 pi): bins per radian of linspace(-pi, pi, R)
-    float g_bins;     // bin-tie guard in bins (angle guard + float32 rounding of the coordinate)
-    float hs;         // R / (2 pi): proj_size / 2 per radian of atan(size / d)
-    float hw, hh;     // W/2, H/2
-    float tguard;     // guard on |dx| - W/2 for the torus image decision
-    float fW, fH, pad;
-    float gam, alp1, bet1, max_th, max_vel;
-};
-
-__device__ inline RepConst rep_const(const VFRep* rp) {
-    RepConst c;
-    c.N = rp->n_agents; c.R = rp->n_rays; c.W = rp->words; c.a0 = rp->agent_offset;
-    c.torus = rp->boundary == ABM_BOUNDARY_INFINITE;
-    c.fov_px0 = rp->fov_px0; c.fov_px1 = rp->fov_px1; c.phi_ok = rp->phi_ok;
-    c.limit = rp->limit_movement; c.field_offset = rp->field_offset;
-    c.inv_step = (float)(1.0 / rp->step);
-    c.g_bins = (float)(kAngleGuardF / rp->step + 1.5e-7 * rp->n_rays);
-    c.hs = (float)((double)rp->n_rays / kTwoPi);
-    c.hw = (float)(rp->width / 2.0);
-    c.hh = (float)(rp->height / 2.0);
-    c.tguard = (float)(1e-6 * (rp->width > rp->height ? rp->width : rp->height) + 1e-4);
-    c.fW = (float)rp->width; c.fH = (float)rp->height; c.pad = (float)rp->pad;
-    c.gam = (float)rp->gam; c.alp1 = (float)rp->alp1; c.bet1 = (float)rp->bet1;
-    c.max_th = (float)rp->max_th; c.max_vel = (float)rp->max_vel;
-    return c;
-}
-
-// Float32 decision of one pair; returns false when the pair must be
-// re-decided in float64 (inside a guard band, or degenerate).  Centre
-// differences of float32 positions are exact or within 1 ulp of the result
-// (Sterbenz), so the angle is within 1e-6 rad of the float64 one.
-__device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
-                                 float ojy, float ojr, const RepConst& C, int* start, int* end,
-                                 int* drawn) {
-    float dx = (ojx - fx) + (ojr - fr);
-    float dy = (ojy - fy) + (ojr - fr);
-    bool ok = true;
-    if (C.torus) {
-        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
-        ok = fabsf(ex) >= C.tguard && fabsf(ey) >= C.tguard;
-        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
-        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
-    }
-    const float d2 = fmaf(dy, dy, dx * dx);
-    const float d = __builtin_amdgcn_sqrtf(d2);
-    const float a = atan2_f32(fmaf(u1x, dy, -u1y * dx), fmaf(u1x, dx, u1y * dy));
-    const float q = atan2_f32(ojr, d);
-    const float xb = (kPiF - a) * C.inv_step;       // bin coordinate of ca = -a
-    const float xfl = floorf(xb);
-    const float fb = xb - xfl;
-    const float half = q * C.hs;                    // proj_size / 2
-    const float hfl = floorf(half);
-    const float hfrac = half - hfl;
-    const float hg = 4e-6f * half + 1e-6f;
-    ok = ok && (d2 > 1e-30f) && fabsf(a) < kPiF - kAngleGuardF && fabsf(fb - 0.5f) >= C.g_bins &&
-         hfrac >= hg && hfrac <= 1.0f - hg;
-    int t = (int)xfl + (fb >= 0.5f ? 1 : 0);
-    t = t < 0 ? 0 : (t > C.R - 1 ? C.R - 1 : t);
-    const int h = (int)hfl;
-    *start = t - h;
-    *end = t + h;
-    *drawn = ((C.fov_px0 < *start && *start < C.fov_px1) ||
-              (C.fov_px0 < *end && *end < C.fov_px1)) ? 1 : 0;
-    return ok;
-}
-
-__device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx, float ojy,
-                                      float ojr, const VFRep* rp, int* start, int* end, int* drawn) {
-    const Focal f = make_focal((double)fx, (double)fy, (double)fth, (double)fr);
-    const PairVF e = pair_vf_exact(f, (double)ojx, (double)ojy, (double)ojr,
-                                   rp->boundary == ABM_BOUNDARY_INFINITE ? 1 : 0, rp->width,
-                                   rp->height, rp->n_rays, rp->step, rp->fov_px0, rp->fov_px1);
-    *start = e.start; *end = e.end; *drawn = e.skipped ? 0 : e.drawn;
-}
 
 // LDS layout: per-agent records as float4 so one 16-byte read serves a pair.
 struct Smem {
@@ -283,7 +109,7 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
         __syncthreads();
         // heading of agent 0 (later headings come with the previous agent's motion)
         float sth, cth;
-        sincosf(S.pose[0].w, &sth, &cth);
+        sincos_f32(S.pose[0].w, &sth, &cth);
 
         for (int k = 0; k < C.N; ++k) {
             for (int w = tid; w < C.W; w += NT) S.F[w] = 0u;
@@ -405,60 +231,18 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
             }
 
             // ------------------------------------------------ motion
-            float vel = dk.x;
-            float dv = 0.f, dpsi = 0.f;
-            if (C.phi_ok) {
-                dv = C.gam * (v0 - vel) + (float)Sv;
-                dpsi = (float)Sp;
-            }
-            float theta = dpsi;
-            if (C.limit && fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
-            float o = wrap_2pi(fth + theta);
-            vel = vel + dv;
-            if (C.limit && fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
+            float vel, o, dv, dpsi;
+            steer(C, fth, dk.x, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
             // one sincos serves this agent's motion (lane 0) and the next
             // agent's heading (lane 1)
             const float ang = (lane == 1 && k + 1 < C.N) ? S.pose[k + 1].w : o;
             float s_l, c_l;
-            sincosf(ang, &s_l, &c_l);
+            sincos_f32(ang, &s_l, &c_l);
             const float so = lane_f32(s_l, 0), co = lane_f32(c_l, 0);
             sth = lane_f32(s_l, 1);
             cth = lane_f32(c_l, 1);
-            float nx = fx + vel * co;
-            float ny = fy - vel * so;
-            const float xc = nx + fr, yc = ny + fr;
-            const float lo_b = C.pad, hx_b = C.pad + C.fW, hy_b = C.pad + C.fH;
-            if (C.torus) {
-                if (xc < lo_b) nx = hx_b - fr;
-                else if (xc > hx_b) nx = lo_b + fr;
-                if (yc < lo_b) ny = hy_b - fr;
-                else if (yc > hy_b) ny = lo_b + fr;
-            } else {
-                if (xc < lo_b) {
-                    nx = lo_b - fr;
-                    if (kHalfPiF <= o && o < kPiF) o -= kHalfPiF;
-                    else if (kPiF <= o && o <= kThreeHalfPiF) o += kHalfPiF;
-                    o = wrap_2pi(o);
-                }
-                if (xc > hx_b) {
-                    nx = hx_b - fr - 1.f;
-                    if (kThreeHalfPiF <= o && o < kTwoPiF) o -= kHalfPiF;
-                    else if (0.f <= o && o <= kHalfPiF) o += kHalfPiF;
-                    o = wrap_2pi(o);
-                }
-                if (yc < lo_b) {
-                    ny = lo_b - fr;
-                    if (kHalfPiF <= o && o <= kPiF) o += kHalfPiF;
-                    else if (0.f <= o && o < kHalfPiF) o -= kHalfPiF;
-                    o = wrap_2pi(o);
-                }
-                if (yc > hy_b) {
-                    ny = hy_b - fr - 1.f;
-                    if (kThreeHalfPiF <= o && o <= kTwoPiF) o += kHalfPiF;
-                    else if (kPiF <= o && o < kThreeHalfPiF) o -= kHalfPiF;
-                    o = wrap_2pi(o);
-                }
-            }
+            float nx, ny;
+            euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
             if (tid == 0) {
                 S.pose[k] = make_float4(nx, ny, fr, o);
                 S.dyn[k].x = vel;
@@ -545,7 +329,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     const int g = C.a0 + (active ? k : 0);
     const float fx = st.x[g], fy = st.y[g], fth = st.theta[g], fr = (float)st.radius[g];
     float sth, cth;
-    sincosf(fth, &sth, &cth);
+    sincos_f32(fth, &sth, &cth);
     const float u1x = cth, u1y = -sth;
     unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
 
@@ -667,54 +451,12 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
         uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
         for (int w = lane; w < C.W; w += 64) dst[w] = F[w];
     }
-    float vel = st.v[g];
-    float dv = 0.f, dpsi = 0.f;
-    if (C.phi_ok) {
-        dv = C.gam * (v0 - vel) + (float)Sv;
-        dpsi = (float)Sp;
-    }
-    float theta = dpsi;
-    if (C.limit && fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
-    float o = wrap_2pi(fth + theta);
-    vel = vel + dv;
-    if (C.limit && fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
+    float vel, o, dv, dpsi;
+    steer(C, fth, st.v[g], v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
     float so, co;
-    sincosf(o, &so, &co);
-    float nx = fx + vel * co;
-    float ny = fy - vel * so;
-    const float xc = nx + fr, yc = ny + fr;
-    const float lo_b = C.pad, hx_b = C.pad + C.fW, hy_b = C.pad + C.fH;
-    if (C.torus) {
-        if (xc < lo_b) nx = hx_b - fr;
-        else if (xc > hx_b) nx = lo_b + fr;
-        if (yc < lo_b) ny = hy_b - fr;
-        else if (yc > hy_b) ny = lo_b + fr;
-    } else {
-        if (xc < lo_b) {
-            nx = lo_b - fr;
-            if (kHalfPiF <= o && o < kPiF) o -= kHalfPiF;
-            else if (kPiF <= o && o <= kThreeHalfPiF) o += kHalfPiF;
-            o = wrap_2pi(o);
-        }
-        if (xc > hx_b) {
-            nx = hx_b - fr - 1.f;
-            if (kThreeHalfPiF <= o && o < kTwoPiF) o -= kHalfPiF;
-            else if (0.f <= o && o <= kHalfPiF) o += kHalfPiF;
-            o = wrap_2pi(o);
-        }
-        if (yc < lo_b) {
-            ny = lo_b - fr;
-            if (kHalfPiF <= o && o <= kPiF) o += kHalfPiF;
-            else if (0.f <= o && o < kHalfPiF) o -= kHalfPiF;
-            o = wrap_2pi(o);
-        }
-        if (yc > hy_b) {
-            ny = hy_b - fr - 1.f;
-            if (kThreeHalfPiF <= o && o <= kTwoPiF) o += kHalfPiF;
-            else if (kPiF <= o && o < kThreeHalfPiF) o -= kHalfPiF;
-            o = wrap_2pi(o);
-        }
-    }
+    sincos_f32(o, &so, &co);
+    float nx, ny;
+    euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
     if (lane == 0) {
         out_pose[g] = make_float4(nx, ny, o, vel);
         if (want_diag) {
@@ -819,6 +561,18 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
 
 }  // namespace
 
+// Replicas per wave of the grouped kernel for this batch, or 0 for the
+// per-replica kernel.  Automatic choice (measured on MI355X with 50-agent,
+// 512-ray sweeps): one replica per wave, four waves per SIMD, unless the batch
+// has enough groups of four to give every SIMD two waves of them.
+int selected_group_size(const abm_vf_batch* b) {
+    if (!b->group_ok || b->group_g < 0) return 0;
+    if (b->group_g > 0) return b->group_g;
+    if (b->waves_per_group > 0) return 0;
+    if (b->gplan[2].n_groups >= 8 * b->n_cu) return 4;
+    return 1;
+}
+
 int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                        uint32_t flags, hipStream_t s) {
     if (b->max_rays > 2048)
@@ -839,6 +593,8 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         if (need <= 16) return launch_wide<16>(b, st, bits, diag, flags, s);
         return launch_wide<32>(b, st, bits, diag, flags, s);
     }
+    const int G = selected_group_size(b);
+    if (G > 0) return launch_vf_group_f32(b, G, st, bits, diag, flags, s);
     int nw = b->waves_per_group;
     if (nw == 0) nw = b->max_agents > 256 ? 4 : (b->max_agents > 96 ? 2 : 1);
     const int need = (b->max_rays + 64 * nw - 1) / (64 * nw);   // rays per lane

@@ -1,0 +1,119 @@
+// vf_group_common.hpp -- wave-level helpers of the grouped fused kernels
+// (G replicas per wave, L = 64 / G lanes per replica).
+#pragma once
+
+#include "vf_f32_common.hpp"
+
+namespace abm {
+namespace f32 {
+
+// Orders the LDS accesses of different lanes of this wave: everything before
+// is complete and visible to every lane of the wave before anything after.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of lane (group base + L - 2) in every lane of the group.
+template <int G>
+__device__ inline float group_bcast_second_last(float v, int gl) {
+    if (G == 4) {   // row_newbcast:14
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x15E, 0xf, 0xf, false));
+    } else if (G == 2) {
+        const float a = lane_f32(v, 30), b = lane_f32(v, 62);
+        return gl == 0 ? a : b;
+    } else {
+        return lane_f32(v, 62);
+    }
+}
+
+// Masks of n < 32 rays starting at bit s of a word (n = 0: nothing): the bits
+// in that word and the bits that spill into the next one.
+__device__ inline void span_masks(int s, int n, uint32_t* lo, uint32_t* hi) {
+    *lo = ((1u << n) - 1u) << s;
+    const int c = s + n - 32;
+    *hi = c > 0 ? (1u << c) - 1u : 0u;
+}
+
+// Sums over each group of L = 64 / G lanes of two values, their DPP steps
+// interleaved (no hazard stalls between dependent DPP instructions).  The
+// totals are valid in the last lane of every group (for G = 4 in every lane
+// of the group).
+template <int G>
+__device__ inline void group_sum2(float& a, float& b) {
+    a += dpp_f32<0xB1>(a); b += dpp_f32<0xB1>(b);
+    a += dpp_f32<0x4E>(a); b += dpp_f32<0x4E>(b);
+    a += dpp_f32<0x141>(a); b += dpp_f32<0x141>(b);
+    a += dpp_f32<0x140>(a); b += dpp_f32<0x140>(b);
+    if (G <= 2) {
+        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x142, 0xa, 0xf, false));
+        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), 0x142, 0xa, 0xf, false));
+        a += ta; b += tb;
+    }
+    if (G == 1) {
+        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x143, 0xc, 0xf, false));
+        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), 0x143, 0xc, 0xf, false));
+        a += ta; b += tb;
+    }
+}
+
+// Four group sums, DPP steps interleaved.
+template <int G>
+__device__ inline void group_sum4(float& a, float& b, float& c, float& d) {
+    group_sum2<G>(a, b);
+    group_sum2<G>(c, d);
+}
+
+// Bits of lane li's ray span [li * RPL, li * RPL + RPL) of the field F of R
+// rays, with the neighbouring bits the edge rule needs, and the edge mask of
+// the span (dPhi alignment rule of vf_supcalc.py:266-274).
+template <int RPL>
+__device__ inline void span_bits(const uint32_t* F, int li, int R, uint64_t* v_out, uint64_t* sp_out) {
+    const int rs = li * RPL;
+    const int rn = rs >= R ? 0 : (R - rs < RPL ? R - rs : RPL);
+    uint32_t vlo = 0u, vhi = 0u, pbit = 0u, nbit = 0u;
+    if (rn > 0) {
+        if (RPL <= 32) {
+            vlo = RPL == 32 ? F[rs >> 5] : ((F[rs >> 5] >> (rs & 31)) & low_mask(RPL));
+        } else {
+            vlo = F[rs >> 5];
+            vhi = F[(rs >> 5) + 1];
+        }
+        pbit = field_bit(F, rs == 0 ? R - 1 : rs - 1);
+        nbit = field_bit(F, rs + rn >= R ? 0 : rs + rn);
+    }
+    const bool case1 = (F[0] & 1u) == 1u && field_bit(F, R - 1) == 0u;
+    const uint64_t m = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
+    const uint64_t v = (((uint64_t)vhi << 32) | vlo) & m;
+    uint64_t sp = 0;
+    if (rn > 0) {
+        if (case1) sp = (v ^ ((v << 1) | pbit)) & m;
+        else sp = (v ^ ((v >> 1) | ((uint64_t)nbit << (rn - 1)))) & m;
+    }
+    *v_out = v;
+    *sp_out = sp;
+}
+
+// The four sums of one lane's span from the nibble tables (entry [q][v][li]:
+// tb, ts point at lane li's column).
+template <int RPL, int L>
+__device__ inline void span_sums(const float2* tb, const float2* ts, uint64_t v, uint64_t sp,
+                                 float* bc, float* bs, float* ec, float* es) {
+    float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+#pragma unroll
+    for (int q = 0; q < RPL / 4; ++q) {
+        const uint32_t vw = q < 8 ? (uint32_t)v : (uint32_t)(v >> 32);
+        const uint32_t sw = q < 8 ? (uint32_t)sp : (uint32_t)(sp >> 32);
+        const int vb = (vw >> (4 * (q & 7))) & 15;
+        const int sb = (sw >> (4 * (q & 7))) & 15;
+        const float2 tbv = tb[(q * 16 + vb) * L];
+        const float2 tsv = ts[(q * 16 + sb) * L];
+        a += tbv.x; b += tbv.y;
+        c += tsv.x; d += tsv.y;
+    }
+    *bc = a; *bs = b; *ec = c; *es = d;
+}
+
+}  // namespace f32
+}  // namespace abm

@@ -212,7 +212,13 @@ class VFSwarm:
         return w.value, ms.value
 
     def set_launch(self, waves_per_group: int):
+        """Per-replica fused kernel with this many waves per replica (0 = automatic)."""
         L.check(self.batch.lib.abm_vf_set_launch(self.batch.handle, waves_per_group))
+
+    def set_grouping(self, replicas_per_wave: int):
+        """Grouped fused kernel with 1, 2 or 4 replicas per wave; -1 selects the
+        per-replica kernel, 0 the automatic choice."""
+        L.check(self.batch.lib.abm_vf_set_grouping(self.batch.handle, replicas_per_wave))
 
     # ----------------------------------------------------------- read back
     def replica_slice(self, b):

@@ -143,11 +143,23 @@ int abm_vf_step_f32(abm_vf_batch* batch, const abm_vf_state_f32* state,
 int abm_vf_f32_fallback_count(abm_vf_batch* batch, int64_t* pairs_total,
                               int64_t* pairs_rechecked, int64_t* pairs_disagree, void* stream);
 
-/* Select the fused kernel's launch shape: waves per workgroup for each size
- * class (0 = autotune result / default). */
+/* Launch shapes of the fused step (no counterpart in the reference, which
+ * has no kernels).  Two kernels implement it: the per-replica kernel, with
+ * 1, 2 or 4 waves cooperating on one replica, and the grouped kernel, with
+ * 1, 2 or 4 replicas advanced in lockstep by one wave (for batches with one
+ * n_rays <= 1024 and at most 256 agents per replica; replicas that differ in
+ * more than N and the ALP0/BET0/V0 defaults never share a wave).
+ * abm_vf_set_launch selects the per-replica kernel with waves_per_group
+ * waves per replica (0 = automatic choice). */
 int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
+/* replicas_per_wave: 1, 2 or 4 = grouped kernel with that many replicas per
+ * wave (ABM_ERR_UNSUPPORTED when the batch does not qualify), -1 = per-replica
+ * kernel, 0 = automatic choice. */
+int abm_vf_set_grouping(abm_vf_batch* batch, int32_t replicas_per_wave);
 /* Time every candidate launch shape on the given state (the state is
- * advanced by the trial steps) and keep the fastest. */
+ * advanced by the trial steps) and keep the fastest.  *chosen: waves per
+ * replica (1, 2, 4) of the per-replica kernel, or -G for the grouped kernel
+ * with G replicas per wave. */
 int abm_vf_autotune(abm_vf_batch* batch, const abm_vf_state_f32* state, int32_t trial_steps,
                     int32_t* chosen_waves, float* chosen_ms, void* stream);
 

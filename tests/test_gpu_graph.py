@@ -50,14 +50,19 @@ def test_graph_replays_match_eager(gpu, precision):
 def test_autotune_and_launch_shapes_agree(gpu):
     sw = sweep(n_side=6)
     w, ms = sw.autotune(trial_steps=2)
-    assert w in (1, 2, 4) and ms > 0
+    assert w in (1, 2, 4, -1, -2, -4) and ms > 0
     outs = []
-    for nw in (1, 2, 4):
+    # per-replica kernel with 1, 2, 4 waves per replica; grouped kernel with
+    # 1, 2, 4 replicas per wave
+    for kind, n in (("launch", 1), ("launch", 2), ("launch", 4), ("group", 1), ("group", 2),
+                    ("group", 4)):
         t = sweep(n_side=6)
-        t.set_launch(nw)
+        t.set_launch(n) if kind == "launch" else t.set_grouping(n)
         t.step(fields=True)
         outs.append((t.fields.clone(), t.state_numpy()))
     for f, st in outs[1:]:
         assert torch.equal(f, outs[0][0])          # identical rasters
         for a, b in zip(st, outs[0][1]):
-            assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)) < 1e-6
+            # float32 sums in different orders: within 2e-6 of each other (each
+            # within 1e-5 of the float64 oracle)
+            assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)) < 2e-6
