@@ -5,13 +5,13 @@
 // vf_sims.py:302) with the replica's poses and its float32 ray table in LDS.
 //
 // Per focal agent k:
-//   pairs   -- lanes over the other agents j.  The centre differences, torus
-//              image decisions and the bin arithmetic are float64 adds and
-//              compares on the float32 state, hence identical to the float64
-//              reference arithmetic; only the transcendentals (atan2 for the
-//              closed angle, atan for the half-width, sqrt) are float32.  When
-//              the float32 angle lies within a guard band of a bin tie or of the
-//              +-pi wrap, or the half-width within its band of an integer, the
+//   pairs   -- lanes over the other agents j.  The centre differences, the
+//              periodic image, atan2 for the closed angle, atan for the
+//              half-width and the bin arithmetic are float32 (fast_pair in
+//              vf_f32_common.hpp, within 1e-6 rad of the exact angle).  When the
+//              float32 angle lies within a guard band of a bin tie or of the
+//              +-pi wrap, the half-width within its band of an integer, or a
+//              difference within its band of the half-arena image switch, the
 //              pair is re-decided with the exact float64 code of vf_exact.hpp
 //              (vf_supcalc.py:55-119), so the raster equals the float64 one.
 //   raster  -- each drawn interval (wrap rules of vf_supcalc.py:122-129) is

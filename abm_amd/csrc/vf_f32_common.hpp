@@ -158,21 +158,32 @@ __device__ inline RepConst rep_const(const VFRep* rp) {
 }
 
 // Float32 decision of one pair; returns false when the pair must be
-// re-decided in float64 (inside a guard band, or degenerate).  Centre
-// differences of float32 positions are exact or within 1 ulp of the result
-// (Sterbenz), so the angle is within 1e-6 rad of the float64 one.
+// re-decided in float64 (inside a guard band, or degenerate).  The centre
+// difference is exact or rounded once relative to its own size: coordinates
+// close together subtract exactly (Sterbenz), distant ones give a large
+// difference, and for the periodic image the shift by W is applied to the
+// coordinate it leaves within a factor two of W (exact) before the final
+// subtraction -- subtracting first would round at the ulp of W, an angle error
+// of ~3e-5 px / d for a wrapped pair at distance d.  With sincos (1.1e-7),
+// cross/dot, atan2_f32 (2e-7) and the rounding of pi - a (2.4e-7) the float32
+// angle stays within 1e-6 rad of the exact one; the float64 reference is within
+// 5e-8 rad of it.
 __device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
                                  float ojy, float ojr, const RepConst& C, int* start, int* end,
                                  int* drawn) {
-    float dx = (ojx - fx) + (ojr - fr);
-    float dy = (ojy - fy) + (ojr - fr);
+    const float dr = ojr - fr;                       // radii are small integers: exact
+    float dx = (ojx - fx) + dr;
+    float dy = (ojy - fy) + dr;
     bool ok = true;
     if (C.torus) {
         // nearest image: shift by W toward the focal when |dx| > W/2 (same for y)
         const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
         ok = (fabsf(ex) >= C.tguard) & (fabsf(ey) >= C.tguard);
-        dx -= ex > 0.f ? copysignf(C.fW, dx) : 0.f;
-        dy -= ey > 0.f ? copysignf(C.fH, dy) : 0.f;
+        const float sx = ex > 0.f ? copysignf(C.fW, dx) : 0.f;
+        const float sy = ey > 0.f ? copysignf(C.fH, dy) : 0.f;
+        // object shifted down by W, or focal shifted down by W (object above)
+        dx = ((ojx - fmaxf(sx, 0.f)) - (fx + fminf(sx, 0.f))) + dr;
+        dy = ((ojy - fmaxf(sy, 0.f)) - (fy + fminf(sy, 0.f))) + dr;
     }
     const float d2 = fmaf(dy, dy, dx * dx);
     const float d = __builtin_amdgcn_sqrtf(d2);

@@ -226,7 +226,7 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
 
             // ------------------------------------------------ sums
             uint64_t v, sp;
-            span_bits<RPL>(Fg, li, C.R, &v, &sp);
+            span_bits<G, RPL>(Fg, li, C.R, &v, &sp);
             if (bits_out && kact) {
                 uint32_t* dst = bits_out + fo + (int64_t)k * C.W;
                 for (int w = li; w < C.W; w += L) dst[w] = Fg[w];

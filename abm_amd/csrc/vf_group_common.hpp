@@ -46,14 +46,16 @@ __device__ inline void group_sum2(float& a, float& b) {
     a += dpp_f32<0x4E>(a); b += dpp_f32<0x4E>(b);
     a += dpp_f32<0x141>(a); b += dpp_f32<0x141>(b);
     a += dpp_f32<0x140>(a); b += dpp_f32<0x140>(b);
+    // masked row broadcasts: rows outside the mask add -0.0 (exact identity)
+    constexpr int kNegZero = (int)0x80000000u;
     if (G <= 2) {
-        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x142, 0xa, 0xf, false));
-        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), 0x142, 0xa, 0xf, false));
+        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(a), 0x142, 0xa, 0xf, false));
+        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(b), 0x142, 0xa, 0xf, false));
         a += ta; b += tb;
     }
     if (G == 1) {
-        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x143, 0xc, 0xf, false));
-        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), 0x143, 0xc, 0xf, false));
+        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(a), 0x143, 0xc, 0xf, false));
+        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(b), 0x143, 0xc, 0xf, false));
         a += ta; b += tb;
     }
 }
@@ -65,25 +67,55 @@ __device__ inline void group_sum4(float& a, float& b, float& c, float& d) {
     group_sum2<G>(c, d);
 }
 
+// Lane value moved by a DPP control (bound_ctrl: lanes without a source read 0).
+template <int CTRL>
+__device__ inline uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+}
+
 // Bits of lane li's ray span [li * RPL, li * RPL + RPL) of the field F of R
 // rays, with the neighbouring bits the edge rule needs, and the edge mask of
-// the span (dPhi alignment rule of vf_supcalc.py:266-274).
-template <int RPL>
+// the span (dPhi alignment rule of vf_supcalc.py:266-274).  When the spans of
+// a group tile the R rays exactly (R = L * RPL) and a group is the wave (G = 1)
+// or a DPP row (G = 4), the neighbouring bits come from the adjacent lanes'
+// spans by DPP rotation instead of LDS reads.
+template <int G, int RPL>
 __device__ inline void span_bits(const uint32_t* F, int li, int R, uint64_t* v_out, uint64_t* sp_out) {
+    constexpr int L = 64 / G;
     const int rs = li * RPL;
     const int rn = rs >= R ? 0 : (R - rs < RPL ? R - rs : RPL);
     uint32_t vlo = 0u, vhi = 0u, pbit = 0u, nbit = 0u;
-    if (rn > 0) {
-        if (RPL <= 32) {
-            vlo = RPL == 32 ? F[rs >> 5] : ((F[rs >> 5] >> (rs & 31)) & low_mask(RPL));
+    bool case1;
+    if ((G == 1 || G == 4) && RPL <= 32 && R == L * RPL) {   // wave-uniform
+        vlo = RPL == 32 ? F[rs >> 5] : ((F[rs >> 5] >> (rs & 31)) & low_mask(RPL));
+        uint32_t prev, next, first, last;
+        if (G == 1) {
+            prev = dpp_u32<0x13C>(vlo);                       // wave_ror:1 (lane 0 <- lane 63)
+            next = dpp_u32<0x134>(vlo);                       // wave_rol:1 (lane 63 <- lane 0)
+            first = (uint32_t)__builtin_amdgcn_readlane((int)vlo, 0);
+            last = (uint32_t)__builtin_amdgcn_readlane((int)vlo, 63);
         } else {
-            vlo = F[rs >> 5];
-            vhi = F[(rs >> 5) + 1];
+            prev = dpp_u32<0x121>(vlo);                       // row_ror:1
+            next = dpp_u32<0x12F>(vlo);                       // row_ror:15
+            first = dpp_u32<0x150>(vlo);                      // row_newbcast:0
+            last = dpp_u32<0x15F>(vlo);                       // row_newbcast:15
         }
-        pbit = field_bit(F, rs == 0 ? R - 1 : rs - 1);
-        nbit = field_bit(F, rs + rn >= R ? 0 : rs + rn);
+        pbit = (prev >> (RPL - 1)) & 1u;
+        nbit = next & 1u;
+        case1 = (first & 1u) == 1u && ((last >> (RPL - 1)) & 1u) == 0u;
+    } else {
+        if (rn > 0) {
+            if (RPL <= 32) {
+                vlo = RPL == 32 ? F[rs >> 5] : ((F[rs >> 5] >> (rs & 31)) & low_mask(RPL));
+            } else {
+                vlo = F[rs >> 5];
+                vhi = F[(rs >> 5) + 1];
+            }
+            pbit = field_bit(F, rs == 0 ? R - 1 : rs - 1);
+            nbit = field_bit(F, rs + rn >= R ? 0 : rs + rn);
+        }
+        case1 = (F[0] & 1u) == 1u && field_bit(F, R - 1) == 0u;
     }
-    const bool case1 = (F[0] & 1u) == 1u && field_bit(F, R - 1) == 0u;
     const uint64_t m = rn >= 64 ? ~0ull : ((1ull << rn) - 1ull);
     const uint64_t v = (((uint64_t)vhi << 32) | vlo) & m;
     uint64_t sp = 0;

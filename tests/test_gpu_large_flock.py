@@ -55,3 +55,21 @@ def test_c5_flock_verify(gpu):
     pairs, rechecked, disagree = sw.fallback_counts()
     assert disagree == 0
     assert pairs == 3 * 8192 * 8191
+
+
+def test_wide_kernel_seam_pairs(gpu):
+    # the wide (synchronous, N > 2048) kernel shares the pair decision: crowd
+    # agents at the periodic seams and check every decision in verify mode
+    from abm_amd.vf import VFConfig, VFSwarm
+    N, side = 2304, 1500.0
+    rng = np.random.RandomState(77)
+    lo, hi = 25.0, 25.0 + side
+    x = np.where(rng.rand(N) < 0.5, rng.uniform(lo - 4, lo + 4, N), rng.uniform(hi - 4, hi + 4, N))
+    y = rng.uniform(30, 30 + side, N)
+    th = rng.uniform(0, 2 * np.pi, N)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    cfg = VFConfig(n_agents=N, n_rays=1024, width=side, height=side, boundary="infinite")
+    sw = VFSwarm([cfg], f32(x), f32(y), f32(th), 0.5, radius=3, precision="f32")
+    sw.step(order="synchronous", verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert pairs == N * (N - 1) and disagree == 0

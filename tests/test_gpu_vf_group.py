@@ -174,3 +174,43 @@ def test_close_pairs_long_intervals(gpu, G):
         sl = sw.replica_slice(b)
         per_agent_check(sw, b, cfg, states[b], tuple(a[sl] for a in after), max_agents=15)
     assert sw.fallback_counts()[2] == 0
+
+
+def seam_batch(B=64, N=50, side=707.0, R=512, seed=1234):
+    """Agents crowded into thin bands along the periodic seams (and their
+    corner): many pairs are images across the seam at a few px."""
+    from abm_amd.vf import VFConfig
+    rng = np.random.RandomState(seed)
+    cfgs, states = [], []
+    for b in range(B):
+        cfgs.append(VFConfig(n_agents=N, n_rays=R, width=side, height=side, boundary="infinite",
+                             ALP0=1.0, BET0=1.0))
+        lo, hi = 30.0 - 5.0, 30.0 + side - 5.0          # positions of centres at the seam
+        x = np.where(rng.rand(N) < 0.5, rng.uniform(lo - 4, lo + 4, N), rng.uniform(hi - 4, hi + 4, N))
+        y = np.where(rng.rand(N) < 0.5, rng.uniform(lo - 4, lo + 4, N), rng.uniform(30, 30 + side, N))
+        th = rng.uniform(0, 2 * np.pi, N)
+        f32 = lambda a: a.astype(np.float32).astype(np.float64)
+        states.append((f32(x), f32(y), f32(th), np.full(N, 0.5)))
+    return cfgs, states
+
+
+@pytest.mark.parametrize("shape", [("launch", 1), ("group", 1), ("group", 4)])
+def test_seam_pairs_decided_like_float64(gpu, shape):
+    # wrapped pairs at a few px: the periodic image difference must be exact
+    # enough that no float32 decision outside the guard bands differs from the
+    # float64 one (verify mode), over several steps
+    cfgs, states = seam_batch()
+    sw = swarm(cfgs, states)
+    sw.set_launch(shape[1]) if shape[0] == "launch" else sw.set_grouping(shape[1])
+    for _ in range(3):
+        sw.step(verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert pairs > 0 and disagree == 0, (pairs, rechecked, disagree)
+    # and the first step's fields against the oracle for a few replicas
+    sw2 = swarm(cfgs[:3], states[:3])
+    sw2.set_launch(shape[1]) if shape[0] == "launch" else sw2.set_grouping(shape[1])
+    sw2.step(fields=True, diag=True)
+    after = sw2.state_numpy()
+    for b in range(3):
+        sl = sw2.replica_slice(b)
+        per_agent_check(sw2, b, cfgs[b], states[b], tuple(a[sl] for a in after), max_agents=10)
