@@ -562,14 +562,26 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
 }  // namespace
 
 // Replicas per wave of the grouped kernel for this batch, or 0 for the
-// per-replica kernel.  Automatic choice (measured on MI355X with 50-agent,
-// 512-ray sweeps): one replica per wave, four waves per SIMD, unless the batch
-// has enough groups of four to give every SIMD two waves of them.
+// per-replica kernel.  Automatic choice (measured on MI355X): batches with at
+// least 8 replicas per CU run grouped, one replica per wave (four waves per
+// SIMD), or four per wave when that still gives every SIMD two waves; smaller
+// batches are latency-bound and take the per-replica kernel with several
+// waves per replica.
 int selected_group_size(const abm_vf_batch* b) {
     if (!b->group_ok || b->group_g < 0) return 0;
     if (b->group_g > 0) return b->group_g;
     if (b->waves_per_group > 0) return 0;
+    if (b->n_replicas < 8 * b->n_cu) return 0;
     if (b->gplan[2].n_groups >= 8 * b->n_cu) return 4;
+    return 1;
+}
+
+// Waves per replica of the per-replica kernel when not set: as many (up to 4)
+// as leave room for all replicas at once (16 waves per CU).
+static int default_waves(const abm_vf_batch* b) {
+    for (int nw : {4, 2}) {
+        if (b->max_agents > 16 * nw && (long)b->n_replicas * nw <= 16L * b->n_cu) return nw;
+    }
     return 1;
 }
 
@@ -596,7 +608,7 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
     const int G = selected_group_size(b);
     if (G > 0) return launch_vf_group_f32(b, G, st, bits, diag, flags, s);
     int nw = b->waves_per_group;
-    if (nw == 0) nw = b->max_agents > 256 ? 4 : (b->max_agents > 96 ? 2 : 1);
+    if (nw == 0) nw = default_waves(b);
     const int need = (b->max_rays + 64 * nw - 1) / (64 * nw);   // rays per lane
     const int rpl = need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 16 ? 16 : 32));
 #define ABM_LAUNCH(NWV, RPLV) \

@@ -56,15 +56,24 @@ def make_workload(n_rep_side=B_SIDE):
     return cfgs, f32(xs), f32(ys), f32(ths)
 
 
-def time_steps(sw, steps, warmup, use_graph=True):
-    """Average kernel time (ms) of `steps` launches measured with events on
-    the launch stream, after `warmup` untimed steps."""
+def time_steps(sw, steps, warmup, use_graph=True, per_graph=10):
+    """Average time (ms) of one step over `steps` steps, after `warmup` untimed
+    steps.  With graphs, `per_graph` consecutive steps are captured in one HIP
+    graph (the step kernels back to back, no host launch between them); each
+    replay is bracketed by events on the launch stream and its time divided by
+    `per_graph`.  Returns (mean, median, wall-clock per step)."""
     import torch
     for _ in range(warmup):
         sw.step()
     torch.cuda.synchronize()
     graph = None
+    reps = max(1, steps)
+    per = 1
     if use_graph:
+        # steps per graph: the largest divisor of `steps` up to per_graph, so
+        # exactly `steps` steps are timed
+        per = max(d for d in range(1, max(1, min(per_graph, steps)) + 1) if steps % d == 0)
+        reps = max(1, steps // per)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
@@ -73,21 +82,22 @@ def time_steps(sw, steps, warmup, use_graph=True):
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            sw.step()
+            for _ in range(per):
+                sw.step()
         torch.cuda.synchronize()
     run = graph.replay if graph is not None else sw.step
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(steps):
+    for i in range(reps):
         starts[i].record()
         run()
         ends[i].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    return float(np.mean(per)), float(np.median(per)), wall * 1e3 / steps
+    per_step = [s.elapsed_time(e) / per for s, e in zip(starts, ends)]
+    return float(np.mean(per_step)), float(np.median(per_step)), wall * 1e3 / (reps * per)
 
 
 def traffic_bytes():
