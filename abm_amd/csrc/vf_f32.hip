@@ -79,7 +79,7 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
     const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     Smem S = carve(smem_d, maxN, RPL * NT, NW, sync_order);
     int loaded_tab = -1;
-    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    uint32_t n_pairs = 0, n_recheck = 0, n_disagree = 0;   // this lane's
 
     for (int idx = blockIdx.x; idx < n_rep; idx += gridDim.x) {
         const VFRep* rp = reps + order[idx];
@@ -123,14 +123,14 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
                 if (j == k) continue;
                 const float4 o = S.obj[j];
                 if (o.x == fx && o.y == fy) continue;          // coincident (:57)
-                int start, end, drawn;
+                int start, end, drawn, why;
                 const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end,
-                                          &drawn);
+                                          &drawn, &why);
                 if (count) ++n_pairs;
                 if (!ok || verify) {
                     const int slot = atomicAdd(S.nre, 1);
-                    S.rlist[slot] = ok ? j : (j | (1 << 30));
-                    S.rfast[slot] = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
+                    S.rlist[slot] = entry_code(j, 0, ok, why);
+                    S.rfast[slot] = fast_code(start, end, drawn, C.R);
                     if (!ok) continue;
                 }
                 if (drawn) raster(S.F, start, end, C.R);
@@ -141,17 +141,15 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
             if (nre > 0) {
                 for (int i = tid; i < nre; i += NT) {
                     const int e = S.rlist[i];
-                    const int j = e & ((1 << 30) - 1);
-                    const bool forced = (e >> 30) & 1;
+                    const int fc = S.rfast[i];
+                    const int j = e & 0xfffff;
                     const float4 o = S.obj[j];
-                    int start, end, drawn;
-                    exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &start, &end, &drawn);
-                    if (forced) {
+                    int dis;
+                    const int r = resolve_queued(e, fc, fx, fy, fth, fr, o.x, o.y, o.z, rp, verify, &dis);
+                    n_disagree += dis;
+                    if (e & kEntryForced) {
                         if (count) ++n_recheck;
-                        if (drawn) raster(S.F, start, end, C.R);
-                    } else {
-                        const int ex = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
-                        if (S.rfast[i] != ex) ++n_disagree;
+                        if (r >= 0) raster(S.F, (r & 0x7fff) - 2 * C.R, ((r >> 15) & 0x7fff) - 2 * C.R, C.R);
                     }
                 }
                 __syncthreads();
@@ -268,9 +266,9 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
         __syncthreads();
     }
     if (count) {
-        if (n_pairs) atomicAdd(&counters[0], n_pairs);
-        if (n_recheck) atomicAdd(&counters[1], n_recheck);
-        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+        if (n_pairs) atomicAdd(&counters[0], (unsigned long long)n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], (unsigned long long)n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], (unsigned long long)n_disagree);
     }
 }
 
@@ -379,11 +377,21 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
             const int jt = e & ((1 << 30) - 1);
             const bool culled = (e >> 30) & 1;
             const float4 o = S.tile[jt];
-            int start, end, drawn;
-            bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn);
+            int start, end, drawn, why;
+            bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
             if (!ok || verify) {
-                int es, ee, ed;
-                exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                int es = 0, ee = 0, ed = 0;
+                if (verify) exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                if (!ok) {
+                    int qs, qe, qd;
+                    decide_queued(entry_code(0, 0, false, why), fast_code(start, end, drawn, C.R), fx,
+                                  fy, fth, fr, o.x, o.y, o.z, rp, &qs, &qe, &qd);
+                    if (!verify) {
+                        es = qs; ee = qe; ed = qd;
+                    } else if ((qd ? fast_code(qs, qe, 0, C.R) : -1) != (ed ? fast_code(es, ee, 0, C.R) : -1)) {
+                        ++n_disagree;   // the queued decision against the float64 replica
+                    }
+                }
                 if (verify) {
                     // a culled pair must have an empty raster; a fast decision must match
                     const bool empty = !ed || es == ee;

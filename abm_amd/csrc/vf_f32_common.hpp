@@ -168,9 +168,14 @@ __device__ inline RepConst rep_const(const VFRep* rp) {
 // cross/dot, atan2_f32 (2e-7) and the rounding of pi - a (2.4e-7) the float32
 // angle stays within 1e-6 rad of the exact one; the float64 reference is within
 // 5e-8 rad of it.
+// *why (when the pair is not ok): kTieOnly when only the bin-tie band was hit
+// (then kTieAbove says the tie is between t and t + 1, else t - 1 and t),
+// otherwise another band or a degenerate pair.
+constexpr int kTieOnly = 1, kTieAbove = 2;
+
 __device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
                                  float ojy, float ojr, const RepConst& C, int* start, int* end,
-                                 int* drawn) {
+                                 int* drawn, int* why) {
     const float dr = ojr - fr;                       // radii are small integers: exact
     float dx = (ojx - fx) + dr;
     float dy = (ojy - fy) + dr;
@@ -199,23 +204,124 @@ __device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float
     const float hg = 4e-6f * half + 1e-6f;
     // bitwise & (no short circuit): the compiler keeps the tests branch-free.
     // Outside the guard |a| < pi - guard, so t = rint(xb) lies in [0, R - 1].
-    ok = ok & (d2 > 1e-30f) & (fabsf(a) < kPiF - kAngleGuardF) & (fabsf(dfr) <= 0.5f - C.g_bins) &
-         (hfrac >= hg) & (hfrac <= 1.0f - hg);
+    const bool other = ok & (d2 > 1e-30f) & (fabsf(a) < kPiF - kAngleGuardF) & (hfrac >= hg) &
+                       (hfrac <= 1.0f - hg);
+    const bool tie_ok = fabsf(dfr) <= 0.5f - C.g_bins;
+    ok = other & tie_ok;
     const int t = (int)tr;
     const int h = (int)hfl;
     *start = t - h;
     *end = t + h;
     *drawn = ((C.fov_px0 < *start) & (*start < C.fov_px1)) | ((C.fov_px0 < *end) & (*end < C.fov_px1));
+    *why = (other & !tie_ok) ? (kTieOnly | (dfr > 0.f ? kTieAbove : 0)) : 0;
     return ok;
 }
 
-static __device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx, float ojy,
-                                      float ojr, const VFRep* rp, int* start, int* end, int* drawn) {
+// start / end / drawn of a VF pair from its bin and half-width (vf_supcalc.py:116-119).
+__device__ inline void vf_interval(int t, int h, const VFRep* rp, int* start, int* end, int* drawn) {
+    *start = t - h;
+    *end = t + h;
+    *drawn = ((rp->fov_px0 < *start && *start < rp->fov_px1) ||
+              (rp->fov_px0 < *end && *end < rp->fov_px1)) ? 1 : 0;
+}
+
+// Flagged-pair queue entries: j | slot << 20 | kEntryTieAbove | kEntryTieOnly |
+// kEntryForced, and the float32 decision (start + 2R) | (end + 2R) << 15 |
+// kFastDrawn.
+constexpr int kEntryTieAbove = 1 << 28, kEntryTieOnly = 1 << 29, kEntryForced = 1 << 30;
+constexpr int kFastDrawn = 1 << 30;
+
+__device__ inline int entry_code(int j, int slot, bool ok, int why) {
+    return j | (slot << 20) | (ok ? 0 : kEntryForced) | ((why & kTieOnly) ? kEntryTieOnly : 0) |
+           ((why & kTieAbove) ? kEntryTieAbove : 0);
+}
+__device__ inline int fast_code(int start, int end, int drawn, int R) {
+    return (start + 2 * R) | ((end + 2 * R) << 15) | (drawn ? kFastDrawn : 0);
+}
+
+// The decision of a queued pair: for a pair flagged only for the bin tie the
+// float64 side test (tie_bin), else (or when that is undecided) the float64
+// replica of the reference arithmetic.  Sets start / end / drawn.
+static __device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx,
+                                               float ojy, float ojr, const VFRep* rp, int* start,
+                                               int* end, int* drawn);
+
+// A queued pair of the fused kernels' float64 lists, resolved out of line:
+// forced entries get their decision (decide_queued); in verify mode every
+// entry is also decided by the float64 replica and a difference is counted.
+// Returns start | end << 15 (+ 2R each) | drawn << 30, or -1 when not drawn;
+// *disagree is set in verify mode.
+static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float fy, float fth, float fr,
+                                                  float ojx, float ojy, float ojr, const VFRep* rp,
+                                                  bool verify, int* disagree);
+
+// The bin of a pair whose float32 angle fell only inside the bin-tie band
+// (why & kTieOnly): which side of the tie between bins b and b + 1 the
+// reference's closed angle lies on, from float64 geometry instead of the
+// reference's arithmetic.  The closed angle of vf_supcalc.py:142-158 is
+// ca = phi' - theta, phi' the direction of the centre difference v with y up
+// (v exact in float64 from the float32 state, periodic image as in
+// vf_supcalc.py:70-83), so with psi = theta + m_b, m_b the midpoint of the
+// linspace values b and b + 1, s = -v_y cos(psi) - v_x sin(psi) =
+// |v| sin(ca - m_b).  The reference's own angle is within 2e-8 rad of the
+// exact one (acos near 0 is the worst case), so when |s| > 1e-7 |v| its argmin
+// picks b + 1 for s > 0 and b for s < 0.  Returns the bin, or -1 when the pair
+// lies within 1e-7 rad of the tie and the float64 replica must decide.
+__device__ __forceinline__ int tie_bin_inl(float fx, float fy, float fth, float fr, float ojx,
+                                           float ojy, float ojr, const VFRep* rp, int b) {
+    const double cx = (double)fx + (double)fr, cy = (double)fy + (double)fr;
+    double ocx = (double)ojx + (double)ojr, ocy = (double)ojy + (double)ojr;
+    double vx = ocx - cx, vy = ocy - cy;
+    if (rp->boundary == ABM_BOUNDARY_INFINITE) {
+        if (fabs(vx) > rp->width / 2.0) ocx += cx < ocx ? -rp->width : (cx > ocx ? rp->width : 0.0);
+        if (fabs(vy) > rp->height / 2.0) ocy += cy < ocy ? -rp->height : (cy > ocy ? rp->height : 0.0);
+        vx = ocx - cx;
+        vy = ocy - cy;
+    }
+    const int R = rp->n_rays;
+    const double m = 0.5 * (linspace_at(b, R, rp->step) + linspace_at(b + 1, R, rp->step));
+    double sp, cp;
+    sincos((double)fth + m, &sp, &cp);
+    const double sv = -vy * cp - vx * sp;
+    const double d2 = vx * vx + vy * vy;
+    if (sv * sv <= 1e-14 * d2) return -1;
+    return sv > 0.0 ? b + 1 : b;
+}
+
+static __device__ __noinline__ int tie_bin(float fx, float fy, float fth, float fr, float ojx,
+                                           float ojy, float ojr, const VFRep* rp, int b) {
+    return tie_bin_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, b);
+}
+
+static __device__ __noinline__ void decide_queued(int e, int fc, float fx, float fy, float fth, float fr,
+                                                  float ojx, float ojy, float ojr, const VFRep* rp,
+                                                  int* start, int* end, int* drawn) {
+    if (e & kEntryTieOnly) {
+        const int R = rp->n_rays;
+        const int fs = (fc & 0x7fff) - 2 * R, fe = ((fc >> 15) & 0x7fff) - 2 * R;
+        const int t32 = (fs + fe) / 2, h = (fe - fs) / 2;
+        const int t = tie_bin(fx, fy, fth, fr, ojx, ojy, ojr, rp, (e & kEntryTieAbove) ? t32 : t32 - 1);
+        if (t >= 0) {
+            vf_interval(t, h, rp, start, end, drawn);
+            return;
+        }
+    }
+    exact_pair(fx, fy, fth, fr, ojx, ojy, ojr, rp, start, end, drawn);
+}
+
+__device__ __forceinline__ void exact_pair_inl(float fx, float fy, float fth, float fr, float ojx,
+                                               float ojy, float ojr, const VFRep* rp, int* start,
+                                               int* end, int* drawn) {
     const Focal f = make_focal((double)fx, (double)fy, (double)fth, (double)fr);
     const PairVF e = pair_vf_exact(f, (double)ojx, (double)ojy, (double)ojr,
                                    rp->boundary == ABM_BOUNDARY_INFINITE ? 1 : 0, rp->width,
                                    rp->height, rp->n_rays, rp->step, rp->fov_px0, rp->fov_px1);
     *start = e.start; *end = e.end; *drawn = e.skipped ? 0 : e.drawn;
+}
+
+static __device__ __noinline__ void exact_pair(float fx, float fy, float fth, float fr, float ojx, float ojy,
+                                               float ojr, const VFRep* rp, int* start, int* end, int* drawn) {
+    exact_pair_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, start, end, drawn);
 }
 
 
@@ -316,6 +422,47 @@ __device__ inline void euler_boundary(const RepConst& C, float fx, float fy, flo
     *o_io = o;
     *nx_out = nx;
     *ny_out = ny;
+}
+
+}  // namespace f32
+}  // namespace abm
+
+namespace abm {
+namespace f32 {
+
+static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float fy, float fth, float fr,
+                                                  float ojx, float ojy, float ojr, const VFRep* rp,
+                                                  bool verify, int* disagree) {
+    // a leaf (the float64 helpers inlined): a call from the kernels then
+    // clobbers only the registers this function itself uses
+    const int R = rp->n_rays;
+    const bool forced = (e & kEntryForced) != 0;
+    int start = 0, end = 0, drawn = 0;
+    bool decided = false;
+    if (forced && (e & kEntryTieOnly)) {
+        const int fs = (fc & 0x7fff) - 2 * R, fe = ((fc >> 15) & 0x7fff) - 2 * R;
+        const int t32 = (fs + fe) / 2, h = (fe - fs) / 2;
+        const int t = tie_bin_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, (e & kEntryTieAbove) ? t32 : t32 - 1);
+        if (t >= 0) {
+            vf_interval(t, h, rp, &start, &end, &drawn);
+            decided = true;
+        }
+    }
+    int xs = 0, xe = 0, xd = 0;
+    if (verify || (forced && !decided)) exact_pair_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, &xs, &xe, &xd);
+    if (forced && !decided) {
+        start = xs; end = xe; drawn = xd;
+    }
+    if (verify) {
+        const int ex = xd ? fast_code(xs, xe, 0, R) : -1;
+        const int got = forced ? (drawn ? fast_code(start, end, 0, R) : -1)
+                               : ((fc & kFastDrawn) ? (fc & ~kFastDrawn) : -1);
+        *disagree = got != ex;
+    } else {
+        *disagree = 0;
+    }
+    if (!forced) return -1;
+    return drawn ? fast_code(start, end, 1, R) : -1;
 }
 
 }  // namespace f32

@@ -90,6 +90,7 @@ __device__ inline void raster_pred(uint32_t* F, int start, int end, int R, bool 
 
 template <int G, int RPL>
 __global__ void __launch_bounds__(64 * (G == 1 ? 8 : 16 / G))
+__attribute__((amdgpu_waves_per_eu(G == 1 ? 4 : 1)))
 vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ groups, int n_groups,
                 const float2* __restrict__ nib, abm_vf_state_f32 st, uint32_t* __restrict__ bits_out,
                 float* __restrict__ diag, uint32_t flags, unsigned long long* __restrict__ counters,
@@ -125,7 +126,8 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
     for (int w = lane; w < G * (W + 1); w += 64) S.F[w] = 0u;
     __syncthreads();
 
-    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    unsigned long long n_pairs = 0;   // wave total
+    uint32_t n_recheck = 0, n_disagree = 0;   // this lane's
     for (int gi = blockIdx.x * nwg + wave; gi < n_groups; gi += gridDim.x * nwg) {
         const int* grp = groups + (size_t)gi * G;
         const VFRep* rp0 = reps + grp[0];
@@ -171,9 +173,6 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
 
         for (int k = 0; k < nmax; ++k) {
             const bool kact = k < myN;
-            const int kn = k + 1 < nmax ? k + 1 : k;
-            const float4 fpn = poseg[kn];
-            const float4 fdn = dyng[kn];
             const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
             const float u1x = cth, u1y = -sth;
 
@@ -185,8 +184,8 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 const int j = li + t * L;
                 const bool act = kact && j < myN && j != k;
                 const float4 o = objg[act ? j : 0];
-                int start, end, drawn;
-                const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn);
+                int start, end, drawn, why;
+                const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
                 const bool live = act && !(o.x == fx && o.y == fy);     // coincident (:57)
                 if (count) n_pairs += __popcll(__ballot(live));         // wave total
                 const bool flag = live && (!ok || verify);
@@ -194,8 +193,8 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 if (fm) {
                     if (flag) {
                         const int slot = nre + (int)__popcll(fm & ((1ull << lane) - 1ull));
-                        S.rlist[slot] = j | (gl << 20) | (ok ? 0 : (1 << 30));
-                        S.rfast[slot] = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
+                        S.rlist[slot] = entry_code(j, gl, ok, why);
+                        S.rfast[slot] = fast_code(start, end, drawn, C.R);
                     }
                     nre += (int)__popcll(fm);
                 }
@@ -205,26 +204,27 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
             if (nre > 0) {
                 for (int i = lane; i < nre; i += 64) {
                     const int e = S.rlist[i];
+                    const int fc = S.rfast[i];
                     const int j = e & 0xfffff;
-                    const int g = (e >> 20) & 0x3ff;
-                    const bool forced = (e >> 30) & 1;
+                    const int g = (e >> 20) & 0xff;
                     const float4 f = S.pose[g * maxN + k];
                     const float4 o = S.obj[g * maxN + j];
                     const VFRep* rpg = reps + grp[g];
-                    int start, end, drawn;
-                    exact_pair(f.x, f.y, f.w, f.z, o.x, o.y, o.z, rpg, &start, &end, &drawn);
-                    if (forced) {
+                    int dis;
+                    const int r = resolve_queued(e, fc, f.x, f.y, f.w, f.z, o.x, o.y, o.z, rpg, verify, &dis);
+                    n_disagree += dis;
+                    if (e & kEntryForced) {
                         if (count) ++n_recheck;
-                        if (drawn) raster(S.F + g * (W + 1), start, end, C.R);
-                    } else {
-                        const int ex = drawn ? ((start + 2 * C.R) | ((end + 2 * C.R) << 15)) : -1;
-                        if (S.rfast[i] != ex) ++n_disagree;
+                        if (r >= 0) raster(S.F + g * (W + 1), (r & 0x7fff) - 2 * C.R, ((r >> 15) & 0x7fff) - 2 * C.R, C.R);
                     }
                 }
                 wave_sync();
             }
 
             // ------------------------------------------------ sums
+            const int kn = k + 1 < nmax ? k + 1 : k;      // next focal agent
+            const float4 fpn = poseg[kn];
+            const float4 fdn = dyng[kn];
             uint64_t v, sp;
             span_bits<G, RPL>(Fg, li, C.R, &v, &sp);
             if (bits_out && kact) {
@@ -286,8 +286,8 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
     }
     if (count) {
         if (n_pairs && lane == 0) atomicAdd(&counters[0], n_pairs);   // n_pairs is a wave total
-        if (n_recheck) atomicAdd(&counters[1], n_recheck);
-        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+        if (n_recheck) atomicAdd(&counters[1], (unsigned long long)n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], (unsigned long long)n_disagree);
     }
 }
 

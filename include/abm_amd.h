@@ -129,7 +129,7 @@ int abm_vf_step_f64(abm_vf_batch* batch, const abm_vf_state_f64* state,
                     uint32_t* field_bits, double* diag, uint32_t flags, void* stream);
 
 /* One step of every replica, fused float32 path: bit-packed fields, float32
- * integrals with compensated sums, float32 poses.  Ray-bin and half-width
+ * integrals from float64-built ray tables, float32 poses.  Ray-bin and half-width
  * decisions are made in float32 with a guard band and re-made with the exact
  * float64 arithmetic whenever the float32 value falls inside it, so the
  * rasterised field equals the float64 one on the same poses. */

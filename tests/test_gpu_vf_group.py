@@ -214,3 +214,50 @@ def test_seam_pairs_decided_like_float64(gpu, shape):
     for b in range(3):
         sl = sw2.replica_slice(b)
         per_agent_check(sw2, b, cfgs[b], states[b], tuple(a[sl] for a in after), max_agents=10)
+
+
+def tie_batch(B=256, N=50, R=512, side=707.0, seed=99):
+    """Agent 0 of each replica sees the other agents at closed angles within
+    ~1e-6 rad of bin ties (midpoints of linspace(-pi, pi, R)), so nearly all
+    its pairs fall inside the float32 tie band."""
+    from abm_amd.vf import VFConfig
+    rng = np.random.RandomState(seed)
+    phis = np.linspace(-np.pi, np.pi, R)
+    mids = 0.5 * (phis[:-1] + phis[1:])
+    cfgs, states = [], []
+    for b in range(B):
+        cfgs.append(VFConfig(n_agents=N, n_rays=R, width=side, height=side, boundary="infinite",
+                             ALP0=1.0, BET0=1.0))
+        th0 = float(np.float32(rng.uniform(0, 2 * np.pi)))
+        c0 = np.array([rng.uniform(200, 500), rng.uniform(200, 500)])     # centre of agent 0
+        ca = rng.choice(mids[5:-5], N - 1) + rng.uniform(-1e-6, 1e-6, N - 1)
+        phi = ca + th0                                                     # world angle, y up
+        d = rng.uniform(20, 150, N - 1)
+        cx = c0[0] + d * np.cos(phi)
+        cy = c0[1] - d * np.sin(phi)
+        x = np.concatenate([[c0[0]], cx]) - 5.0
+        y = np.concatenate([[c0[1]], cy]) - 5.0
+        th = np.concatenate([[th0], rng.uniform(0, 2 * np.pi, N - 1)])
+        f32 = lambda a: a.astype(np.float32).astype(np.float64)
+        states.append((f32(x), f32(y), f32(th), np.full(N, 0.5)))
+    return cfgs, states
+
+
+@pytest.mark.parametrize("shape", [("launch", 1), ("group", 1), ("group", 4)])
+def test_tie_pairs_decided_like_float64(gpu, shape):
+    # the float64 side test of tie-flagged pairs against the float64 replica of
+    # the reference arithmetic, thousands of times (verify mode), and the fields
+    # of the agents that see the ties against the oracle
+    cfgs, states = tie_batch()
+    sw = swarm(cfgs, states)
+    sw.set_launch(shape[1]) if shape[0] == "launch" else sw.set_grouping(shape[1])
+    sw.step(verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert rechecked > 256 * 30 and disagree == 0, (pairs, rechecked, disagree)
+    sw2 = swarm(cfgs[:4], states[:4])
+    sw2.set_launch(shape[1]) if shape[0] == "launch" else sw2.set_grouping(shape[1])
+    sw2.step(fields=True, diag=True)
+    after = sw2.state_numpy()
+    for b in range(4):
+        sl = sw2.replica_slice(b)
+        per_agent_check(sw2, b, cfgs[b], states[b], tuple(a[sl] for a in after), max_agents=3)
