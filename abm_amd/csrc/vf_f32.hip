@@ -124,8 +124,11 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
                 const float4 o = S.obj[j];
                 if (o.x == fx && o.y == fy) continue;          // coincident (:57)
                 int start, end, drawn, why;
-                const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end,
-                                          &drawn, &why);
+                bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
+                if (!ok && (why & kTieOnly)) {   // rare
+                    ok = tie_in_place(why, fx, fy, fth, fr, o.x, o.y, o.z, rp, C, &start, &end, &drawn);
+                    if (count && ok) ++n_recheck;   // a float64 decision too
+                }
                 if (count) ++n_pairs;
                 if (!ok || verify) {
                     const int slot = atomicAdd(S.nre, 1);

@@ -246,11 +246,12 @@ static __device__ __noinline__ void exact_pair(float fx, float fy, float fth, fl
                                                float ojy, float ojr, const VFRep* rp, int* start,
                                                int* end, int* drawn);
 
-// A queued pair of the fused kernels' float64 lists, resolved out of line:
-// forced entries get their decision (decide_queued); in verify mode every
-// entry is also decided by the float64 replica and a difference is counted.
-// Returns start | end << 15 (+ 2R each) | drawn << 30, or -1 when not drawn;
-// *disagree is set in verify mode.
+// A queued pair of the fused kernels' float64 lists, resolved out of line by
+// the float64 replica (pairs flagged only for the bin tie are decided in the
+// pair pass by tie_in_place).  Forced entries return their decision, start |
+// end << 15 (+ 2R each) | drawn << 30, or -1 when not drawn; in verify mode an
+// unforced entry's float32 decision is compared and *disagree set on a
+// difference.
 static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float fy, float fth, float fr,
                                                   float ojx, float ojy, float ojr, const VFRep* rp,
                                                   bool verify, int* disagree);
@@ -267,6 +268,35 @@ static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float
 // exact one (acos near 0 is the worst case), so when |s| > 1e-7 |v| its argmin
 // picks b + 1 for s > 0 and b for s < 0.  Returns the bin, or -1 when the pair
 // lies within 1e-7 rad of the tie and the float64 replica must decide.
+// sin and cos in float64 for |x| < 1e5: Cody-Waite reduction by pi/2 with a
+// two-part constant (the first part has 33 significant bits, so k * part is
+// exact for |k| < 2^20 and x - k * part is exact), then the Taylor series of
+// sin to r^11 and of cos to r^12 on |r| <= pi/4 (truncation below 1e-11).
+// Returns false (outputs unset) for larger arguments or NaN.
+__device__ __forceinline__ bool sincos_f64_small(double x, double* s, double* c) {
+    if (!(fabs(x) < 1e5)) return false;
+    const double k = rint(x * 0.63661977236758138);
+    double r = fma(-k, 1.5707963267341256, x);
+    r = fma(-k, 6.077100506506192e-11, r);
+    const double r2 = r * r;
+    double ps = fma(r2, -2.5052108385441720e-08, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -1.9841269841269841e-04);
+    ps = fma(r2, ps, 8.3333333333333333e-03);
+    ps = fma(r2, ps, -1.6666666666666667e-01);
+    const double sn = fma(r * r2, ps, r);
+    double pc = fma(r2, 2.0876756987868099e-09, -2.7557319223985891e-07);
+    pc = fma(r2, pc, 2.4801587301587302e-05);
+    pc = fma(r2, pc, -1.3888888888888889e-03);
+    pc = fma(r2, pc, 4.1666666666666667e-02);
+    pc = fma(r2, pc, -0.5);
+    const double cn = fma(r2, pc, 1.0);
+    const int q = (int)k;
+    const double s0 = (q & 1) ? cn : sn, c0 = (q & 1) ? sn : cn;
+    *s = (q & 2) ? -s0 : s0;
+    *c = ((q + 1) & 2) ? -c0 : c0;
+    return true;
+}
+
 __device__ __forceinline__ int tie_bin_inl(float fx, float fy, float fth, float fr, float ojx,
                                            float ojy, float ojr, const VFRep* rp, int b) {
     const double cx = (double)fx + (double)fr, cy = (double)fy + (double)fr;
@@ -281,7 +311,7 @@ __device__ __forceinline__ int tie_bin_inl(float fx, float fy, float fth, float 
     const int R = rp->n_rays;
     const double m = 0.5 * (linspace_at(b, R, rp->step) + linspace_at(b + 1, R, rp->step));
     double sp, cp;
-    sincos((double)fth + m, &sp, &cp);
+    if (!sincos_f64_small((double)fth + m, &sp, &cp)) return -1;
     const double sv = -vy * cp - vx * sp;
     const double d2 = vx * vx + vy * vy;
     if (sv * sv <= 1e-14 * d2) return -1;
@@ -291,6 +321,23 @@ __device__ __forceinline__ int tie_bin_inl(float fx, float fy, float fth, float 
 static __device__ __noinline__ int tie_bin(float fx, float fy, float fth, float fr, float ojx,
                                            float ojy, float ojr, const VFRep* rp, int b) {
     return tie_bin_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, b);
+}
+
+// A pair flagged only for the bin tie (fast_pair's *why & kTieOnly) decided
+// in place by the float64 side test (an out-of-line leaf taking and returning
+// values only, so the hot loop keeps start / end / drawn in registers): on
+// success sets start / end / drawn and returns true; otherwise the pair is
+// left to the float64 replica.
+__device__ __forceinline__ bool tie_in_place(int why, float fx, float fy, float fth, float fr, float ojx,
+                                             float ojy, float ojr, const VFRep* rp, const RepConst& C,
+                                             int* start, int* end, int* drawn) {
+    const int t32 = (*start + *end) / 2, h = (*end - *start) / 2;
+    const int t = tie_bin(fx, fy, fth, fr, ojx, ojy, ojr, rp, (why & kTieAbove) ? t32 : t32 - 1);
+    if (t < 0) return false;
+    *start = t - h;
+    *end = t + h;
+    *drawn = ((C.fov_px0 < *start) & (*start < C.fov_px1)) | ((C.fov_px0 < *end) & (*end < C.fov_px1));
+    return true;
 }
 
 static __device__ __noinline__ void decide_queued(int e, int fc, float fx, float fy, float fth, float fr,
@@ -433,36 +480,21 @@ namespace f32 {
 static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float fy, float fth, float fr,
                                                   float ojx, float ojy, float ojr, const VFRep* rp,
                                                   bool verify, int* disagree) {
-    // a leaf (the float64 helpers inlined): a call from the kernels then
+    // a leaf (the float64 replica inlined): a call from the kernels then
     // clobbers only the registers this function itself uses
     const int R = rp->n_rays;
     const bool forced = (e & kEntryForced) != 0;
-    int start = 0, end = 0, drawn = 0;
-    bool decided = false;
-    if (forced && (e & kEntryTieOnly)) {
-        const int fs = (fc & 0x7fff) - 2 * R, fe = ((fc >> 15) & 0x7fff) - 2 * R;
-        const int t32 = (fs + fe) / 2, h = (fe - fs) / 2;
-        const int t = tie_bin_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, (e & kEntryTieAbove) ? t32 : t32 - 1);
-        if (t >= 0) {
-            vf_interval(t, h, rp, &start, &end, &drawn);
-            decided = true;
-        }
-    }
     int xs = 0, xe = 0, xd = 0;
-    if (verify || (forced && !decided)) exact_pair_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, &xs, &xe, &xd);
-    if (forced && !decided) {
-        start = xs; end = xe; drawn = xd;
-    }
+    exact_pair_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, &xs, &xe, &xd);
     if (verify) {
         const int ex = xd ? fast_code(xs, xe, 0, R) : -1;
-        const int got = forced ? (drawn ? fast_code(start, end, 0, R) : -1)
-                               : ((fc & kFastDrawn) ? (fc & ~kFastDrawn) : -1);
-        *disagree = got != ex;
+        const int got = (fc & kFastDrawn) ? (fc & ~kFastDrawn) : -1;
+        *disagree = !forced && got != ex;
     } else {
         *disagree = 0;
     }
     if (!forced) return -1;
-    return drawn ? fast_code(start, end, 1, R) : -1;
+    return xd ? fast_code(xs, xe, 1, R) : -1;
 }
 
 }  // namespace f32

@@ -172,6 +172,7 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
         sincos_f32(fp.w, &sth, &cth);
 
         for (int k = 0; k < nmax; ++k) {
+            progress_priority(k, nmax);
             const bool kact = k < myN;
             const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
             const float u1x = cth, u1y = -sth;
@@ -185,8 +186,15 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 const bool act = kact && j < myN && j != k;
                 const float4 o = objg[act ? j : 0];
                 int start, end, drawn, why;
-                const bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
+                bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
                 const bool live = act && !(o.x == fx && o.y == fy);     // coincident (:57)
+                const bool tie = live && !ok && (why & kTieOnly);
+                if (__ballot(tie)) {   // rare
+                    if (tie) {
+                        ok = tie_in_place(why, fx, fy, fth, fr, o.x, o.y, o.z, reps + rid, C, &start, &end, &drawn);
+                        if (count && ok) ++n_recheck;   // a float64 decision too
+                    }
+                }
                 if (count) n_pairs += __popcll(__ballot(live));         // wave total
                 const bool flag = live && (!ok || verify);
                 const uint64_t fm = __ballot(flag);

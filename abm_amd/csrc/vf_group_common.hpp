@@ -15,6 +15,28 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Issue priority from progress through the n focal agents: 3 for the first
+// half, 2 to three quarters, 1 to fifteen sixteenths, then 0.  At equal
+// priority a SIMD's arbiter favours its oldest wave, so waves that share a
+// SIMD would finish one after another and the last would run alone,
+// latency-bound; lowering the priority of waves that have advanced lets the
+// younger ones keep up, and the short last stage keeps the staggered finish
+// short.  Changes only at stage boundaries (k wave-uniform).
+__device__ inline int progress_stage(int k, int n) {
+    const int e = 16 * k;
+    return e < 8 * n ? 0 : (e < 12 * n ? 1 : (e < 15 * n ? 2 : 3));
+}
+__device__ inline void progress_priority(int k, int n) {
+    const int q = progress_stage(k, n);
+    if (k > 0 && q == progress_stage(k - 1, n)) return;
+    switch (q) {
+        case 0: __builtin_amdgcn_s_setprio(3); break;
+        case 1: __builtin_amdgcn_s_setprio(2); break;
+        case 2: __builtin_amdgcn_s_setprio(1); break;
+        default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
 // Value of lane (group base + L - 2) in every lane of the group.
 template <int G>
 __device__ inline float group_bcast_second_last(float v, int gl) {
@@ -46,17 +68,30 @@ __device__ inline void group_sum2(float& a, float& b) {
     a += dpp_f32<0x4E>(a); b += dpp_f32<0x4E>(b);
     a += dpp_f32<0x141>(a); b += dpp_f32<0x141>(b);
     a += dpp_f32<0x140>(a); b += dpp_f32<0x140>(b);
-    // masked row broadcasts: rows outside the mask add -0.0 (exact identity)
-    constexpr int kNegZero = (int)0x80000000u;
-    if (G <= 2) {
-        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(a), 0x142, 0xa, 0xf, false));
-        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(b), 0x142, 0xa, 0xf, false));
-        a += ta; b += tb;
-    }
+    // Masked row broadcasts as single DPP adds: rows outside row_mask are not
+    // written and keep their value.  The compiler does not fuse a masked
+    // broadcast move into the add (no exact float identity for the unwritten
+    // lanes), so these are written out; each DPP read of a register written
+    // by the VALU in the two instructions before it gets its two wait states
+    // from the s_nop in front of the pair, and the trailing s_nop covers a
+    // DPP read of the results right after the block.
     if (G == 1) {
-        const float ta = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(a), 0x143, 0xc, 0xf, false));
-        const float tb = __int_as_float(__builtin_amdgcn_update_dpp(kNegZero, __float_as_int(b), 0x143, 0xc, 0xf, false));
-        a += ta; b += tb;
+        asm volatile(
+            "s_nop 1\n\t"
+            "v_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "v_add_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+            "v_add_f32_dpp %1, %1, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+            "s_nop 1"
+            : "+v"(a), "+v"(b));
+    } else if (G == 2) {
+        asm volatile(
+            "s_nop 1\n\t"
+            "v_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "v_add_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "s_nop 1"
+            : "+v"(a), "+v"(b));
     }
 }
 
