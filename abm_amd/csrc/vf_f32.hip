@@ -123,20 +123,17 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
                 if (j == k) continue;
                 const float4 o = S.obj[j];
                 if (o.x == fx && o.y == fy) continue;          // coincident (:57)
-                int start, end, drawn, why;
-                bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
-                if (!ok && (why & kTieOnly)) {   // rare
-                    ok = tie_in_place(why, fx, fy, fth, fr, o.x, o.y, o.z, rp, C, &start, &end, &drawn);
-                    if (count && ok) ++n_recheck;   // a float64 decision too
-                }
+                PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
                 if (count) ++n_pairs;
-                if (!ok || verify) {
+                if (p.tie && tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, rp, C))   // rare
+                    if (count) ++n_recheck;   // a float64 decision too
+                if (!p.ok || verify) {
                     const int slot = atomicAdd(S.nre, 1);
-                    S.rlist[slot] = entry_code(j, 0, ok, why);
-                    S.rfast[slot] = fast_code(start, end, drawn, C.R);
-                    if (!ok) continue;
+                    S.rlist[slot] = entry_code(j, 0, p.ok);
+                    S.rfast[slot] = fast_code(p.start, p.end, p.drawn, C.R);
+                    if (!p.ok) continue;
                 }
-                if (drawn) raster(S.F, start, end, C.R);
+                if (p.drawn) raster(S.F, p.start, p.end, C.R);
             }
             __syncthreads();
             // ------------------------------------- pairs re-decided in float64
@@ -380,35 +377,26 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
             const int jt = e & ((1 << 30) - 1);
             const bool culled = (e >> 30) & 1;
             const float4 o = S.tile[jt];
-            int start, end, drawn, why;
-            bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
-            if (!ok || verify) {
+            PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
+            if (p.tie && tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, rp, C))   // rare
+                if (count) ++n_recheck;   // a float64 decision too
+            if (!p.ok || verify) {
                 int es = 0, ee = 0, ed = 0;
-                if (verify) exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
-                if (!ok) {
-                    int qs, qe, qd;
-                    decide_queued(entry_code(0, 0, false, why), fast_code(start, end, drawn, C.R), fx,
-                                  fy, fth, fr, o.x, o.y, o.z, rp, &qs, &qe, &qd);
-                    if (!verify) {
-                        es = qs; ee = qe; ed = qd;
-                    } else if ((qd ? fast_code(qs, qe, 0, C.R) : -1) != (ed ? fast_code(es, ee, 0, C.R) : -1)) {
-                        ++n_disagree;   // the queued decision against the float64 replica
-                    }
-                }
+                exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
                 if (verify) {
-                    // a culled pair must have an empty raster; a fast decision must match
+                    // a culled pair must have an empty raster; a float32 decision must match
                     const bool empty = !ed || es == ee;
                     if ((culled && !empty) ||
-                        (!culled && ok && (es != start || ee != end || ed != drawn)))
+                        (!culled && p.ok && (es != p.start || ee != p.end || ed != (int)p.drawn)))
                         ++n_disagree;
                 }
-                if (!ok) {
+                if (!p.ok) {
                     if (count) ++n_recheck;
-                    start = es; end = ee; drawn = ed;
+                    p.start = es; p.end = ee; p.drawn = ed != 0;
                 }
             }
             if (culled) continue;
-            if (drawn) raster(F, start, end, C.R);
+            if (p.drawn) raster(F, p.start, p.end, C.R);
         }
     }
     __syncthreads();

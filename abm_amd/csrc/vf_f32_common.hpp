@@ -168,14 +168,19 @@ __device__ inline RepConst rep_const(const VFRep* rp) {
 // cross/dot, atan2_f32 (2e-7) and the rounding of pi - a (2.4e-7) the float32
 // angle stays within 1e-6 rad of the exact one; the float64 reference is within
 // 5e-8 rad of it.
-// *why (when the pair is not ok): kTieOnly when only the bin-tie band was hit
-// (then kTieAbove says the tie is between t and t + 1, else t - 1 and t),
-// otherwise another band or a degenerate pair.
-constexpr int kTieOnly = 1, kTieAbove = 2;
+// The result: .ok false when the pair must be decided in float64; .tie when
+// only the bin-tie band was hit (the tie is between t and t + 1 for dfr > 0,
+// else t - 1 and t), for tie_in_place.
+struct PairF32 {
+    int start, end;   // t - h, t + h
+    bool drawn;       // an end inside the FOV (vf_supcalc.py:116-119)
+    bool ok;          // outside every guard band: the float32 decision stands
+    bool tie;         // only the bin-tie band was hit
+    float dfr;        // bin coordinate minus its nearest bin
+};
 
-__device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
-                                 float ojy, float ojr, const RepConst& C, int* start, int* end,
-                                 int* drawn, int* why) {
+__device__ inline PairF32 fast_pair(float u1x, float u1y, float fx, float fy, float fr, float ojx,
+                                   float ojy, float ojr, const RepConst& C) {
     const float dr = ojr - fr;                       // radii are small integers: exact
     float dx = (ojx - fx) + dr;
     float dy = (ojy - fy) + dr;
@@ -184,11 +189,13 @@ __device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float
         // nearest image: shift by W toward the focal when |dx| > W/2 (same for y)
         const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
         ok = (fabsf(ex) >= C.tguard) & (fabsf(ey) >= C.tguard);
-        const float sx = ex > 0.f ? copysignf(C.fW, dx) : 0.f;
-        const float sy = ey > 0.f ? copysignf(C.fH, dy) : 0.f;
-        // object shifted down by W, or focal shifted down by W (object above)
-        dx = ((ojx - fmaxf(sx, 0.f)) - (fx + fminf(sx, 0.f))) + dr;
-        dy = ((ojy - fmaxf(sy, 0.f)) - (fy + fminf(sy, 0.f))) + dr;
+        // object shifted down by W (object above), or focal shifted down by W
+        // (object below); selects, no min / max (those would canonicalise)
+        const bool wx = ex > 0.f, px = dx > 0.f, wy = ey > 0.f, py = dy > 0.f;
+        const float ox = (wx & px) ? C.fW : 0.f, fxs = (wx & !px) ? C.fW : 0.f;
+        const float oy = (wy & py) ? C.fH : 0.f, fys = (wy & !py) ? C.fH : 0.f;
+        dx = ((ojx - ox) - (fx - fxs)) + dr;
+        dy = ((ojy - oy) - (fy - fys)) + dr;
     }
     const float d2 = fmaf(dy, dy, dx * dx);
     const float d = __builtin_amdgcn_sqrtf(d2);
@@ -207,14 +214,16 @@ __device__ inline bool fast_pair(float u1x, float u1y, float fx, float fy, float
     const bool other = ok & (d2 > 1e-30f) & (fabsf(a) < kPiF - kAngleGuardF) & (hfrac >= hg) &
                        (hfrac <= 1.0f - hg);
     const bool tie_ok = fabsf(dfr) <= 0.5f - C.g_bins;
-    ok = other & tie_ok;
     const int t = (int)tr;
     const int h = (int)hfl;
-    *start = t - h;
-    *end = t + h;
-    *drawn = ((C.fov_px0 < *start) & (*start < C.fov_px1)) | ((C.fov_px0 < *end) & (*end < C.fov_px1));
-    *why = (other & !tie_ok) ? (kTieOnly | (dfr > 0.f ? kTieAbove : 0)) : 0;
-    return ok;
+    PairF32 p;
+    p.start = t - h;
+    p.end = t + h;
+    p.drawn = ((C.fov_px0 < p.start) & (p.start < C.fov_px1)) | ((C.fov_px0 < p.end) & (p.end < C.fov_px1));
+    p.ok = other & tie_ok;
+    p.tie = other & !tie_ok;
+    p.dfr = dfr;
+    return p;
 }
 
 // start / end / drawn of a VF pair from its bin and half-width (vf_supcalc.py:116-119).
@@ -225,15 +234,14 @@ __device__ inline void vf_interval(int t, int h, const VFRep* rp, int* start, in
               (rp->fov_px0 < *end && *end < rp->fov_px1)) ? 1 : 0;
 }
 
-// Flagged-pair queue entries: j | slot << 20 | kEntryTieAbove | kEntryTieOnly |
-// kEntryForced, and the float32 decision (start + 2R) | (end + 2R) << 15 |
-// kFastDrawn.
-constexpr int kEntryTieAbove = 1 << 28, kEntryTieOnly = 1 << 29, kEntryForced = 1 << 30;
+// Queued-pair entries: j | slot << 20 | kEntryForced (the float64 replica
+// decides) and the float32 decision (start + 2R) | (end + 2R) << 15 |
+// kFastDrawn (compared in verify mode).
+constexpr int kEntryForced = 1 << 30;
 constexpr int kFastDrawn = 1 << 30;
 
-__device__ inline int entry_code(int j, int slot, bool ok, int why) {
-    return j | (slot << 20) | (ok ? 0 : kEntryForced) | ((why & kTieOnly) ? kEntryTieOnly : 0) |
-           ((why & kTieAbove) ? kEntryTieAbove : 0);
+__device__ inline int entry_code(int j, int slot, bool ok) {
+    return j | (slot << 20) | (ok ? 0 : kEntryForced);
 }
 __device__ inline int fast_code(int start, int end, int drawn, int R) {
     return (start + 2 * R) | ((end + 2 * R) << 15) | (drawn ? kFastDrawn : 0);
@@ -323,37 +331,21 @@ static __device__ __noinline__ int tie_bin(float fx, float fy, float fth, float 
     return tie_bin_inl(fx, fy, fth, fr, ojx, ojy, ojr, rp, b);
 }
 
-// A pair flagged only for the bin tie (fast_pair's *why & kTieOnly) decided
-// in place by the float64 side test (an out-of-line leaf taking and returning
-// values only, so the hot loop keeps start / end / drawn in registers): on
-// success sets start / end / drawn and returns true; otherwise the pair is
-// left to the float64 replica.
-__device__ __forceinline__ bool tie_in_place(int why, float fx, float fy, float fth, float fr, float ojx,
-                                             float ojy, float ojr, const VFRep* rp, const RepConst& C,
-                                             int* start, int* end, int* drawn) {
-    const int t32 = (*start + *end) / 2, h = (*end - *start) / 2;
-    const int t = tie_bin(fx, fy, fth, fr, ojx, ojy, ojr, rp, (why & kTieAbove) ? t32 : t32 - 1);
+// A pair of fast_pair's with .tie set decided in place by the float64 side
+// test (an out-of-line leaf taking and returning values only, so the hot loop
+// keeps the pair in registers): on success sets start / end / drawn and .ok;
+// otherwise the pair is left to the float64 replica.  Returns .ok.
+__device__ __forceinline__ bool tie_in_place(PairF32& p, float fx, float fy, float fth, float fr,
+                                             float ojx, float ojy, float ojr, const VFRep* rp,
+                                             const RepConst& C) {
+    const int t32 = (p.start + p.end) / 2, h = (p.end - p.start) / 2;
+    const int t = tie_bin(fx, fy, fth, fr, ojx, ojy, ojr, rp, p.dfr > 0.f ? t32 : t32 - 1);
     if (t < 0) return false;
-    *start = t - h;
-    *end = t + h;
-    *drawn = ((C.fov_px0 < *start) & (*start < C.fov_px1)) | ((C.fov_px0 < *end) & (*end < C.fov_px1));
+    p.start = t - h;
+    p.end = t + h;
+    p.drawn = ((C.fov_px0 < p.start) & (p.start < C.fov_px1)) | ((C.fov_px0 < p.end) & (p.end < C.fov_px1));
+    p.ok = true;
     return true;
-}
-
-static __device__ __noinline__ void decide_queued(int e, int fc, float fx, float fy, float fth, float fr,
-                                                  float ojx, float ojy, float ojr, const VFRep* rp,
-                                                  int* start, int* end, int* drawn) {
-    if (e & kEntryTieOnly) {
-        const int R = rp->n_rays;
-        const int fs = (fc & 0x7fff) - 2 * R, fe = ((fc >> 15) & 0x7fff) - 2 * R;
-        const int t32 = (fs + fe) / 2, h = (fe - fs) / 2;
-        const int t = tie_bin(fx, fy, fth, fr, ojx, ojy, ojr, rp, (e & kEntryTieAbove) ? t32 : t32 - 1);
-        if (t >= 0) {
-            vf_interval(t, h, rp, start, end, drawn);
-            return;
-        }
-    }
-    exact_pair(fx, fy, fth, fr, ojx, ojy, ojr, rp, start, end, drawn);
 }
 
 __device__ __forceinline__ void exact_pair_inl(float fx, float fy, float fth, float fr, float ojx,

@@ -171,8 +171,9 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
         float sth, cth;
         sincos_f32(fp.w, &sth, &cth);
 
+        const ProgressStages stages = progress_stages(nmax);
         for (int k = 0; k < nmax; ++k) {
-            progress_priority(k, nmax);
+            progress_priority(k, stages);
             const bool kact = k < myN;
             const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
             const float u1x = cth, u1y = -sth;
@@ -185,28 +186,23 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 const int j = li + t * L;
                 const bool act = kact && j < myN && j != k;
                 const float4 o = objg[act ? j : 0];
-                int start, end, drawn, why;
-                bool ok = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C, &start, &end, &drawn, &why);
+                PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
                 const bool live = act && !(o.x == fx && o.y == fy);     // coincident (:57)
-                const bool tie = live && !ok && (why & kTieOnly);
-                if (__ballot(tie)) {   // rare
-                    if (tie) {
-                        ok = tie_in_place(why, fx, fy, fth, fr, o.x, o.y, o.z, reps + rid, C, &start, &end, &drawn);
-                        if (count && ok) ++n_recheck;   // a float64 decision too
-                    }
-                }
                 if (count) n_pairs += __popcll(__ballot(live));         // wave total
-                const bool flag = live && (!ok || verify);
-                const uint64_t fm = __ballot(flag);
-                if (fm) {
+                // pairs outside the float32 decision (rare), every pair in verify mode
+                if (__ballot(live && !p.ok) || verify) {
+                    if (live && p.tie && tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, reps + rid, C))
+                        if (count) ++n_recheck;   // a float64 decision too
+                    const bool flag = live && (!p.ok || verify);
+                    const uint64_t fm = __ballot(flag);
                     if (flag) {
                         const int slot = nre + (int)__popcll(fm & ((1ull << lane) - 1ull));
-                        S.rlist[slot] = entry_code(j, gl, ok, why);
-                        S.rfast[slot] = fast_code(start, end, drawn, C.R);
+                        S.rlist[slot] = entry_code(j, gl, p.ok);
+                        S.rfast[slot] = fast_code(p.start, p.end, p.drawn, C.R);
                     }
                     nre += (int)__popcll(fm);
                 }
-                raster_pred(Fg, start, end, C.R, live && ok && drawn != 0);
+                raster_pred(Fg, p.start, p.end, C.R, live && p.ok && p.drawn);
             }
             wave_sync();
             if (nre > 0) {

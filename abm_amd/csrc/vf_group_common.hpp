@@ -22,19 +22,18 @@ __device__ inline void wave_sync() {
 // latency-bound; lowering the priority of waves that have advanced lets the
 // younger ones keep up, and the short last stage keeps the staggered finish
 // short.  Changes only at stage boundaries (k wave-uniform).
-__device__ inline int progress_stage(int k, int n) {
-    const int e = 16 * k;
-    return e < 8 * n ? 0 : (e < 12 * n ? 1 : (e < 15 * n ? 2 : 3));
+struct ProgressStages {
+    int k1, k2, k3;   // first focal agent of stages 1, 2, 3
+};
+__device__ inline ProgressStages progress_stages(int n) {
+    // stage(k) = 0 for 16 k < 8 n, 1 for 16 k < 12 n, 2 for 16 k < 15 n, else 3
+    return {(8 * n + 15) / 16, (12 * n + 15) / 16, (15 * n + 15) / 16};
 }
-__device__ inline void progress_priority(int k, int n) {
-    const int q = progress_stage(k, n);
-    if (k > 0 && q == progress_stage(k - 1, n)) return;
-    switch (q) {
-        case 0: __builtin_amdgcn_s_setprio(3); break;
-        case 1: __builtin_amdgcn_s_setprio(2); break;
-        case 2: __builtin_amdgcn_s_setprio(1); break;
-        default: __builtin_amdgcn_s_setprio(0); break;
-    }
+__device__ inline void progress_priority(int k, const ProgressStages& ps) {
+    if (k == 0) __builtin_amdgcn_s_setprio(3);
+    else if (k == ps.k3) __builtin_amdgcn_s_setprio(0);
+    else if (k == ps.k2) __builtin_amdgcn_s_setprio(1);
+    else if (k == ps.k1) __builtin_amdgcn_s_setprio(2);
 }
 
 // Value of lane (group base + L - 2) in every lane of the group.
@@ -50,12 +49,20 @@ __device__ inline float group_bcast_second_last(float v, int gl) {
     }
 }
 
-// Masks of n < 32 rays starting at bit s of a word (n = 0: nothing): the bits
-// in that word and the bits that spill into the next one.
+// ((1 << width) - 1) << offset for width, offset in [0, 31], truncated to
+// 32 bits (one v_bfm_b32; the compiler does not form it from the shifts).
+__device__ inline uint32_t bit_field_mask(uint32_t width, uint32_t offset) {
+    uint32_t r;
+    asm("v_bfm_b32 %0, %1, %2" : "=v"(r) : "v"(width), "v"(offset));
+    return r;
+}
+
+// Masks of n < 32 rays starting at bit s < 32 of a word (n = 0: nothing): the
+// bits in that word and the bits that spill into the next one.
 __device__ inline void span_masks(int s, int n, uint32_t* lo, uint32_t* hi) {
-    *lo = ((1u << n) - 1u) << s;
-    const int c = s + n - 32;
-    *hi = c > 0 ? (1u << c) - 1u : 0u;
+    *lo = bit_field_mask((uint32_t)n, (uint32_t)s);
+    const uint32_t e = (uint32_t)(s + n);
+    *hi = bit_field_mask(e > 32u ? e - 32u : 0u, 0u);
 }
 
 // Sums over each group of L = 64 / G lanes of two values, their DPP steps
