@@ -410,10 +410,13 @@ __device__ inline void steer(const RepConst& C, float fth, float vel, float v0, 
         dpsi = Sp;
     }
     float theta = dpsi;
-    if (C.limit && fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
-    *o = wrap_2pi(fth + theta);
     vel = vel + dv;
-    if (C.limit && fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
+    if (C.limit) {   // uniform; kept a branch (the empty asm stops if-conversion)
+        asm volatile("");
+        if (fabsf(theta) > C.max_th) theta = theta < 0.f ? -C.max_th : C.max_th;
+        if (fabsf(vel) > C.max_vel) vel = vel < 0.f ? -C.max_vel : C.max_vel;
+    }
+    *o = wrap_2pi(fth + theta);
     *vel_new = vel;
     *dv_out = dv;
     *dpsi_out = dpsi;

@@ -239,13 +239,12 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
             for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
             float bc, bs, ec, es;
             span_sums<RPL, L>(tb, ts, v, sp, &bc, &bs, &ec, &es);
-            float Sv, Sp, Bc = 0.f, Bs = 0.f, Ec = 0.f, Es = 0.f;
+            float Sv, Sp, Bc = bc, Bs = bs, Ec = ec, Es = es;   // group totals in diag mode
             if (!want_diag) {
                 Sv = fd.y * (-bc) + (fd.y * C.alp1) * ec;
                 Sp = fd.z * (-bs) + (fd.z * C.bet1) * es;
                 group_sum2<G>(Sv, Sp);
             } else {
-                Bc = bc; Bs = bs; Ec = ec; Es = es;
                 group_sum2<G>(Bc, Bs);
                 group_sum2<G>(Ec, Es);
                 Sv = fd.y * -Bc + (fd.y * C.alp1) * Ec;

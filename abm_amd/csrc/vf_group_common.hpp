@@ -171,20 +171,32 @@ __device__ inline void span_bits(const uint32_t* F, int li, int R, uint64_t* v_o
 
 // The four sums of one lane's span from the nibble tables (entry [q][v][li]:
 // tb, ts point at lane li's column).
+// Bits [off, off + 4) of x (one v_bfe_u32; left to itself the compiler turns
+// the extract-then-scale of the table index into shift, and, add).
+__device__ inline uint32_t nibble(uint32_t x, uint32_t off) {
+    uint32_t r;
+    asm("v_bfe_u32 %0, %1, %2, 4" : "=v"(r) : "v"(x), "s"(off));
+    return r;
+}
+
 template <int RPL, int L>
 __device__ inline void span_sums(const float2* tb, const float2* ts, uint64_t v, uint64_t sp,
                                  float* bc, float* bs, float* ec, float* es) {
-    float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+    float a, b, c, d;
 #pragma unroll
     for (int q = 0; q < RPL / 4; ++q) {
         const uint32_t vw = q < 8 ? (uint32_t)v : (uint32_t)(v >> 32);
         const uint32_t sw = q < 8 ? (uint32_t)sp : (uint32_t)(sp >> 32);
-        const int vb = (vw >> (4 * (q & 7))) & 15;
-        const int sb = (sw >> (4 * (q & 7))) & 15;
-        const float2 tbv = tb[(q * 16 + vb) * L];
-        const float2 tsv = ts[(q * 16 + sb) * L];
-        a += tbv.x; b += tbv.y;
-        c += tsv.x; d += tsv.y;
+        const uint32_t vb = nibble(vw, 4 * (q & 7));
+        const uint32_t sb = nibble(sw, 4 * (q & 7));
+        const float2 tbv = (tb + q * 16 * L)[vb * L];
+        const float2 tsv = (ts + q * 16 * L)[sb * L];
+        if (q == 0) {   // no 0.f + x (not an identity for -0.0, so it stays an add)
+            a = tbv.x; b = tbv.y; c = tsv.x; d = tsv.y;
+        } else {
+            a += tbv.x; b += tbv.y;
+            c += tsv.x; d += tsv.y;
+        }
     }
     *bc = a; *bs = b; *ec = c; *es = d;
 }
