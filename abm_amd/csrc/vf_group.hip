@@ -236,7 +236,11 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 for (int w = li; w < C.W; w += L) dst[w] = Fg[w];
             }
             wave_sync();
-            for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
+            if (W + 1 <= L) {   // wave-uniform
+                if (li < W + 1) Fg[li] = 0u;
+            } else {
+                for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
+            }
             float bc, bs, ec, es;
             span_sums<RPL, L>(tb, ts, v, sp, &bc, &bs, &ec, &es);
             float Sv, Sp, Bc = bc, Bs = bs, Ec = ec, Es = es;   // group totals in diag mode

@@ -145,6 +145,14 @@ __device__ inline void span_bits(const uint32_t* F, int li, int R, uint64_t* v_o
         pbit = (prev >> (RPL - 1)) & 1u;
         nbit = next & 1u;
         case1 = (first & 1u) == 1u && ((last >> (RPL - 1)) & 1u) == 0u;
+        // every span is full (rn = RPL): 32-bit masks, constant shifts
+        const uint32_t m = low_mask(RPL);
+        uint32_t sp32;
+        if (case1) sp32 = (vlo ^ ((vlo << 1) | pbit)) & m;   // wave-uniform
+        else sp32 = vlo ^ ((vlo >> 1) | (nbit << (RPL - 1)));
+        *v_out = vlo;
+        *sp_out = sp32;
+        return;
     } else {
         if (rn > 0) {
             if (RPL <= 32) {
