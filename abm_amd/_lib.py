@@ -90,6 +90,15 @@ class ForageState(C.Structure):
         "novelty", "mode", "overriding_mode", "env_status", "exploited_patch_id", "radius")]
 
 
+class CSParams(C.Structure):
+    _fields_ = [
+        ("n_rays", C.c_int32), ("has_max_proj_size", C.c_int32),
+        ("fov0", C.c_double), ("fov1", C.c_double), ("max_proj_size", C.c_double),
+    ]
+
+
+CS_ROWS, CS_SUM, CS_MAX = 0, 1, 2   # ABM_CS_* modes
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _D = C.c_double
@@ -120,6 +129,7 @@ _SIGS = {
     "abm_forage_step_f64": (C.c_int, [_P, C.POINTER(ForageState), _P, _P, _P, _P, _P, _P]),
     "abm_forage_projection": (C.c_int, [_I32, _D, _D, _I32, _D, _I32, _D, _D, _D, _D,
                                         _P, _P, _I32, _I32, _P, _P]),
+    "abm_cs_projection": (C.c_int, [C.POINTER(CSParams), _I32, _P, _P, _P, _P, _P, _I32, _P, _P]),
 }
 
 _lib = None

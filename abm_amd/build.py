@@ -36,6 +36,7 @@ UNITS = {
     "vf_group.hip": ["-ffp-contract=fast"],
     "forage.hip": ["-ffp-contract=off"],
     "mt19937.hip": [],
+    "cs.hip": ["-ffp-contract=off"],
 }
 
 

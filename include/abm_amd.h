@@ -26,6 +26,8 @@
  *   abm_vf_state_variables   -> vf_supcalc.VSWRM_flocking_state_variables and
  *                               dPhi_V_of (vf_supcalc.py:161-277)
  *   abm_forage_projection    -> Agent.projection_field (abm/agent/agent.py:457-597)
+ *   abm_cs_projection        -> cooperative-signalling projection_field and the
+ *                               two combines of CSAgent (see below)
  */
 #ifndef ABM_AMD_H
 #define ABM_AMD_H
@@ -183,6 +185,40 @@ int abm_vf_state_variables(int32_t n_rays, int32_t n, const uint32_t* bits, cons
                            const double* alp0, const double* bet0, const double* v0,
                            double gam, double alp1, double bet1,
                            double* out, int8_t* dphi, void* stream);
+
+/* ------------------------------------------------ cooperative signalling */
+/* Field parameters of cs_supcalc.projection_field
+ * (abm/projects/cooperative_signaling/cs_agent/cs_supcalc.py:204-295). */
+typedef struct abm_cs_params {
+    int32_t n_rays;              /* v_field_resolution                          */
+    int32_t has_max_proj_size;   /* 0: max_proj_size=None (every size is valid) */
+    double  fov0, fov1;          /* fov                                         */
+    double  max_proj_size;       /* objects with proj_size > this are not drawn */
+} abm_cs_params;
+
+#define ABM_CS_ROWS 0   /* the per-object rows projection_field returns        */
+#define ABM_CS_SUM  1   /* rows summed over objects, / norm[f]
+                           (CSAgent.calc_crowing_density_proj, cs_agent.py:83-104) */
+#define ABM_CS_MAX  2   /* maximum over rows, 0 without objects
+                           (CSAgent.calc_others_signaling_density_proj,
+                            cs_agent.py:106-131)                               */
+
+/* projection_field of the cooperative-signalling agent for n_focal focal
+ * agents, each with its own object list, in float64 with the reference's
+ * operation order.  Replaces cs_supcalc.projection_field (rows) and the
+ * visual_field.sum(axis=0) / len(agents) and visual_field.max(axis=0) steps of
+ * CSAgent.  All arrays are device arrays:
+ *   focal[n_focal][4]: x, y, radius, orientation (position = top-left);
+ *   obj_offset[n_focal + 1]: objects of focal f are obj[obj_offset[f] ..
+ *     obj_offset[f + 1]), each {x, y} (object_positions);
+ *   meters (optional): one per object (object_meters; NULL = None);
+ *   norm (optional, ABM_CS_SUM): divisor per focal agent (len(agents));
+ *   out: ABM_CS_ROWS: [total objects][n_rays] rows in the returned order
+ *     (flipped, FOV-masked); ABM_CS_SUM / ABM_CS_MAX: [n_focal][n_rays].
+ * n_rays in [2, 8192]. */
+int abm_cs_projection(const abm_cs_params* params, int32_t n_focal, const double* focal,
+                      const int64_t* obj_offset, const double* obj, const double* meters,
+                      const double* norm, int32_t mode, double* out, void* stream);
 
 /* ---------------------------------------------------------- foraging */
 /* One foraging replica (Simulation + Agent parameters, abm/simulation/sims.py:60-161,
