@@ -50,8 +50,12 @@ class CSProjection:
     optional (n_obj,) float64.  Positions are sprite top-left corners.
     """
 
-    def __init__(self, v_field_resolution: int, fov=(-np.pi, np.pi), max_proj_size=None):
+    def __init__(self, v_field_resolution: int, fov=(-np.pi, np.pi), max_proj_size=None,
+                 check_offsets: bool = True):
         self.R = int(v_field_resolution)
+        # the offsets bound every object read of the kernel; a caller that
+        # built them itself may skip the host check (a device-to-host copy)
+        self.check_offsets = check_offsets
         self.params = L.CSParams(self.R, 0 if max_proj_size is None else 1, float(fov[0]),
                                  float(fov[1]), 0.0 if max_proj_size is None else float(max_proj_size))
         self.lib = L.load()
@@ -71,10 +75,10 @@ class CSProjection:
             raise L.AbmError(L.ABM_ERR_INVALID, "meters must be float64 (n_obj,)")
         if norm is not None and (norm.dtype != torch.float64 or tuple(norm.shape) != (n_focal,)):
             raise L.AbmError(L.ABM_ERR_INVALID, "norm must be float64 (n_focal,)")
-        # the offsets bound every object read: check them on the host
-        off = obj_offset.cpu()
-        if int(off[0]) != 0 or int(off[-1]) != n_obj or bool((off[1:] < off[:-1]).any()):
-            raise L.AbmError(L.ABM_ERR_INVALID, "obj_offset must rise from 0 to n_obj")
+        if self.check_offsets:
+            off = obj_offset.cpu()
+            if int(off[0]) != 0 or int(off[-1]) != n_obj or bool((off[1:] < off[:-1]).any()):
+                raise L.AbmError(L.ABM_ERR_INVALID, "obj_offset must rise from 0 to n_obj")
         if mode == L.CS_ROWS and n_obj == 0:
             return out                       # no rows to write
         obj_ptr = L.ptr(obj) if n_obj > 0 else None

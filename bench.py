@@ -213,6 +213,8 @@ def other_workload(args):
                         "traffic": None,
                         "note": "nominal all-pairs ops; pairs beyond the exact visibility cut-off are dropped"},
               pair_ray_evals_per_s=Nl * (Nl - 1) * Rl / (ms * 1e-3))
+    elif args.workload == "cs":
+        return cs_workload(args)
     else:   # mixed ragged batch: even replicas forage, odd replicas flock
         rng = np.random.RandomState(20000)
         vf_cfgs, fg_cfgs, vx, fx_ = [], [], [], []
@@ -258,6 +260,63 @@ def other_workload(args):
               foraging={"agent_steps_per_s": n_fg / (ms_fg * 1e-3), "ms_per_step_wall": ms_fg})
 
 
+def cs_workload(args):
+    """Cooperative-signalling perception of every agent of 1000 replicas
+    (N=50, R=1200, FOV 0.45 pi, 600x600 arena): the crowding density (sum over
+    the 49 others, max_proj_size = 0.5 R) and the signalling density (maximum
+    over 20 current signallers and 4 remembered steps of 20, decayed meters)."""
+    import torch
+    from abm_amd.cs import CSProjection
+    B, N, R, Wc, S, MEM = 1000, 50, 1200, 600.0, 20, 5
+    fov = (-0.45 * np.pi, 0.45 * np.pi)
+    rng = np.random.RandomState(77)
+    dev = torch.device("cuda")
+    pos = rng.uniform(PAD, PAD + Wc, (B, N, 2))
+    th = rng.uniform(0, 2 * np.pi, (B, N))
+    focal = np.concatenate([pos, np.full((B, N, 1), 10.0), th[..., None]], axis=2).reshape(-1, 4)
+    # crowding: the others of the same replica
+    idx = np.array([[j for j in range(N) if j != k] for k in range(N)])
+    crowd_obj = pos[:, idx, :].reshape(-1, 2)
+    crowd_off = np.arange(B * N + 1, dtype=np.int64) * (N - 1)
+    # signalling: S current positions and MEM-1 remembered steps (the same for
+    # every agent of a replica; the kernel takes one list per focal agent)
+    sig_pos = rng.uniform(PAD, PAD + Wc, (B, MEM, S, 2))
+    sig_met = rng.uniform(0, 1, (B, MEM, S)) * (1 - np.arange(MEM) / MEM)[None, :, None]
+    sig_obj = np.repeat(sig_pos.reshape(B, 1, MEM * S, 2), N, axis=1).reshape(-1, 2)
+    sig_m = np.repeat(sig_met.reshape(B, 1, MEM * S), N, axis=1).reshape(-1)
+    sig_off = np.arange(B * N + 1, dtype=np.int64) * (MEM * S)
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    F, CO, CF, SO, SM, SF = (T(focal), T(crowd_obj), T(crowd_off, torch.int64), T(sig_obj), T(sig_m),
+                             T(sig_off, torch.int64))
+    norm = torch.full((B * N,), float(N), dtype=torch.float64, device=dev)
+    crowd = CSProjection(R, fov, 0.5 * R, check_offsets=False)
+    sig = CSProjection(R, fov, None, check_offsets=False)
+    out_c = torch.empty((B * N, R), dtype=torch.float64, device=dev)
+    out_s = torch.empty((B * N, R), dtype=torch.float64, device=dev)
+    s = torch.cuda.current_stream()
+
+    def step():
+        crowd._run(1, F, CF, CO, None, norm, out_c, s)
+        sig._run(2, F, SF, SO, SM, None, out_s, s)
+    for _ in range(max(2, args.warmup // 10)):
+        step()
+    torch.cuda.synchronize()
+    k = max(3, min(args.steps, 20))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(k):
+        step()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / k
+    cells = B * N * ((N - 1) + MEM * S) * R
+    _line(args, f"cooperative-signalling perception: {B} replicas x N={N} x R={R}, FOV 0.45 pi, "
+                f"crowding (49 others) + signalling ({MEM * S} remembered signallers)",
+          B * N / (ms * 1e-3), ms, "f64", {"replicas": B, "agents_per_replica": N, "rays": R},
+          object_ray_cells_per_s=cells / (ms * 1e-3),
+          note="agent-steps/s counts focal agents whose two projections were computed")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,11 +326,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "single", "mixed", "large"],
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "single", "mixed", "large", "cs"],
                     help="sweep: 4096 replicas x N=50 x R=512 (default, the headline); single: "
                          "one replica N=100, R=1024, 1000x1000 torus; mixed: 2000 ragged replicas, "
                          "half foraging, half flocking; large: one flock N=8192, R=2048, "
-                         "4000x4000 torus, synchronous order")
+                         "4000x4000 torus, synchronous order; cs: cooperative-signalling "
+                         "perception of 1000 replicas x N=50 x R=1200")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")

@@ -215,7 +215,7 @@ typedef struct abm_cs_params {
  *   norm (optional, ABM_CS_SUM): divisor per focal agent (len(agents));
  *   out: ABM_CS_ROWS: [total objects][n_rays] rows in the returned order
  *     (flipped, FOV-masked); ABM_CS_SUM / ABM_CS_MAX: [n_focal][n_rays].
- * n_rays in [2, 8192]. */
+ * n_rays in [2, 7936]. */
 int abm_cs_projection(const abm_cs_params* params, int32_t n_focal, const double* focal,
                       const int64_t* obj_offset, const double* obj, const double* meters,
                       const double* norm, int32_t mode, double* out, void* stream);

@@ -148,3 +148,40 @@ def test_argument_errors(gpu):
         proj.sum(F, torch.tensor([0, 3, 2], dtype=torch.int64, device=dev), O)
     with pytest.raises(L.AbmError):
         cs.CSProjection(1).rows(F, torch.tensor([0, 1, 3], dtype=torch.int64, device=dev), O)
+
+
+def test_ordered_combines_match_numpy(gpu):
+    # the object-order fold: a SUM with meters, and a MAX where a meter is
+    # negative, -0.0 or NaN (numpy's reduce semantics, NaN propagating)
+    from abm_amd import cs
+    rng = np.random.RandomState(23)
+    R, fov = 600, (-0.8 * PI, 0.8 * PI)
+    proj = cs.CSProjection(R, fov, None)
+    dev = torch.device("cuda")
+    focal, offs, objs, mets, cases = [], [0], [], [], []
+    for f in range(40):
+        n = int(rng.choice([1, 4, 30, 260]))
+        pos, r, th, o, m = _case(rng, R, n, fov, True, None)
+        if f % 4 == 1:
+            m[0] = -0.3
+        elif f % 4 == 2:
+            m[min(2, n - 1)] = -0.0
+        elif f % 4 == 3:
+            m[n // 2] = float("nan")
+        focal.append([pos[0], pos[1], r, th])
+        objs += o
+        mets += m
+        offs.append(offs[-1] + n)
+        cases.append((pos, r, th, o, m))
+    F = torch.tensor(focal, dtype=torch.float64, device=dev)
+    O = torch.tensor(np.array(objs).reshape(-1, 2), dtype=torch.float64, device=dev)
+    off = torch.tensor(offs, dtype=torch.int64, device=dev)
+    M = torch.tensor(mets, dtype=torch.float64, device=dev)
+    N = torch.tensor([7.0] * len(cases), dtype=torch.float64, device=dev)
+    got_sum = proj.sum(F, off, O, N, M).cpu().numpy()
+    got_max = proj.max(F, off, O, M).cpu().numpy()
+    for f, (pos, r, th, o, m) in enumerate(cases):
+        rows = co.projection_field(fov, R, pos, r, th, o, m)
+        for got, ref in ((got_sum[f], rows.sum(axis=0) / 7.0), (got_max[f], rows.max(axis=0))):
+            assert np.array_equal(np.isnan(got), np.isnan(ref)), f
+            assert np.array_equal(np.nan_to_num(got, nan=3.0), np.nan_to_num(ref, nan=3.0)), f
