@@ -130,6 +130,7 @@ _SIGS = {
     "abm_forage_projection": (C.c_int, [_I32, _D, _D, _I32, _D, _I32, _D, _D, _D, _D,
                                         _P, _P, _I32, _I32, _P, _P]),
     "abm_cs_projection": (C.c_int, [C.POINTER(CSParams), _I32, _P, _P, _P, _P, _P, _I32, _P, _P]),
+    "abm_ensemble_stats": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32, _D, _D, _P, _P]),
 }
 
 _lib = None

@@ -37,6 +37,7 @@ UNITS = {
     "forage.hip": ["-ffp-contract=off"],
     "mt19937.hip": [],
     "cs.hip": ["-ffp-contract=off"],
+    "stats.hip": ["-ffp-contract=off"],
 }
 
 

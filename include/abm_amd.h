@@ -220,6 +220,27 @@ int abm_cs_projection(const abm_cs_params* params, int32_t n_focal, const double
                       const int64_t* obj_offset, const double* obj, const double* meters,
                       const double* norm, int32_t mode, double* out, void* stream);
 
+/* ------------------------------------------------- ensemble statistics */
+/* Per-replica statistics of the current state, as abm/loader/data_loader.py
+ * computes them offline from the logged trajectories (positions logged as
+ * int(position), abm/monitoring/ifdb.py:83-84):
+ *   out[4 b + 0] polarisation |sum (cos th, sin th)| / N (calculate_polarization,
+ *                data_loader.py:1761-1836; NaN orientations skipped as nansum does)
+ *   out[4 b + 1] mean inter-individual distance over pairs i < j
+ *                (calculate_interindividual_distance, :1367-1459)
+ *   out[4 b + 2] mean over agents of the nearest-neighbour distance
+ *   out[4 b + 3] the value calculate_mean_NN_dist (:1461-1486) returns for the
+ *                step: min_j>0 d(0, j) / N, because its nanmin runs over an iid
+ *                matrix that holds distances only above the diagonal
+ * agent_offset[n_replicas + 1] (device): agents of replica b are
+ * [agent_offset[b], agent_offset[b + 1]).  x, y, theta: device arrays of
+ * float32 (state_f32 = 1) or float64.  periodic: distances on a torus of
+ * int(width) x int(height) (supcalc.distance_torus).  A replica with one
+ * agent gives NaN distances. */
+int abm_ensemble_stats(int32_t n_replicas, const int32_t* agent_offset, const void* x, const void* y,
+                       const void* theta, int32_t state_f32, int32_t periodic, double width,
+                       double height, double* out, void* stream);
+
 /* ---------------------------------------------------------- foraging */
 /* One foraging replica (Simulation + Agent parameters, abm/simulation/sims.py:60-161,
  * abm/contrib/decision_params.py:13-42, movement_params.py:13-24). */
