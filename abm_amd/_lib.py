@@ -99,6 +99,12 @@ class CSParams(C.Structure):
 
 CS_ROWS, CS_SUM, CS_MAX = 0, 1, 2   # ABM_CS_* modes
 
+
+class LogBuffers(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "posx", "posy", "orientation", "velocity", "w", "u", "I_priv", "collected_r", "mode",
+        "expl_patch_id", "up", "down")]
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _D = C.c_double
@@ -131,6 +137,8 @@ _SIGS = {
                                         _P, _P, _I32, _I32, _P, _P]),
     "abm_cs_projection": (C.c_int, [C.POINTER(CSParams), _I32, _P, _P, _P, _P, _P, _I32, _P, _P]),
     "abm_ensemble_stats": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32, _D, _D, _P, _P]),
+    "abm_log_record": (C.c_int, [C.POINTER(LogBuffers), C.c_int64, C.c_int64, _I32, _P, _P, _P, _P,
+                                 _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P]),
 }
 
 _lib = None

@@ -38,6 +38,7 @@ UNITS = {
     "mt19937.hip": [],
     "cs.hip": ["-ffp-contract=off"],
     "stats.hip": ["-ffp-contract=off"],
+    "agentlog.hip": [],
 }
 
 

@@ -241,6 +241,31 @@ int abm_ensemble_stats(int32_t n_replicas, const int32_t* agent_offset, const vo
                        const void* theta, int32_t state_f32, int32_t periodic, double width,
                        double height, double* out, void* stream);
 
+/* ------------------------------------------------------- agent records */
+/* Device arrays of a run's per-step agent records, indexed [t][agent]
+ * (edges [t][agent][ceil(R/32)] bits).  The optional arrays may be NULL. */
+typedef struct abm_log_buffers {
+    int32_t* posx; int32_t* posy;            /* int(position): INT32_MIN if not finite */
+    double* orientation; double* velocity;
+    double* w; double* u; double* I_priv; double* collected_r;   /* optional */
+    int32_t* mode; int32_t* expl_patch_id;                        /* optional */
+    uint32_t* up; uint32_t* down;            /* optional: vfield_up / vfield_down as bits */
+} abm_log_buffers;
+
+/* One step of ifdb.save_agent_data_RAM (abm/monitoring/ifdb.py:55-96) for
+ * n_agents agents into slot t (0 <= t < capacity) of buf.  x, y, theta, v:
+ * float32 (state_f32 = 1) or float64 device arrays; the foraging arrays are
+ * optional.  field_bits (optional): ceil(n_rays/32) words per agent, the
+ * agent's soc_v_field in output order (raw_raster = 0, the foraging field) or
+ * its raw raster (raw_raster = 1, the VSWRM field, soc_v_field =
+ * flip(raster)); bit k of up / down is set where np.roll(f, 1)[k] < f[k] /
+ * > f[k]. */
+int abm_log_record(const abm_log_buffers* buf, int64_t t, int64_t capacity, int32_t n_agents,
+                   const void* x, const void* y, const void* theta, const void* v, int32_t state_f32,
+                   const double* w, const double* u, const double* I_priv, const double* collected_r,
+                   const int32_t* mode, const int32_t* expl_patch_id, const uint32_t* field_bits,
+                   int32_t n_rays, int32_t raw_raster, void* stream);
+
 /* ---------------------------------------------------------- foraging */
 /* One foraging replica (Simulation + Agent parameters, abm/simulation/sims.py:60-161,
  * abm/contrib/decision_params.py:13-42, movement_params.py:13-24). */
