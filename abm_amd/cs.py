@@ -87,19 +87,26 @@ class CSProjection:
                                            mode, L.ptr(out), L.stream_ptr(stream)))
         return out
 
-    def rows(self, focal, obj_offset, obj, meters=None, stream=None):
+    def _out(self, rows, focal, out):
+        if out is None:
+            return torch.empty((rows, self.R), dtype=torch.float64, device=focal.device)
+        if out.dtype != torch.float64 or tuple(out.shape) != (rows, self.R) or not out.is_contiguous():
+            raise L.AbmError(L.ABM_ERR_INVALID, f"out must be a contiguous float64 ({rows}, {self.R})")
+        return out
+
+    def rows(self, focal, obj_offset, obj, meters=None, out=None, stream=None):
         """Per-object rows as projection_field returns them (flipped, FOV-masked)."""
-        out = torch.empty((int(obj.shape[0]), self.R), dtype=torch.float64, device=focal.device)
+        out = self._out(int(obj.shape[0]), focal, out)
         return self._run(L.CS_ROWS, focal, obj_offset, obj, meters, None, out, stream)
 
-    def sum(self, focal, obj_offset, obj, norm=None, meters=None, stream=None):
+    def sum(self, focal, obj_offset, obj, norm=None, meters=None, out=None, stream=None):
         """Rows summed over each focal agent's objects, divided by norm."""
-        out = torch.empty((int(focal.shape[0]), self.R), dtype=torch.float64, device=focal.device)
+        out = self._out(int(focal.shape[0]), focal, out)
         return self._run(L.CS_SUM, focal, obj_offset, obj, meters, norm, out, stream)
 
-    def max(self, focal, obj_offset, obj, meters=None, stream=None):
+    def max(self, focal, obj_offset, obj, meters=None, out=None, stream=None):
         """Maximum over each focal agent's rows (zeros without objects)."""
-        out = torch.empty((int(focal.shape[0]), self.R), dtype=torch.float64, device=focal.device)
+        out = self._out(int(focal.shape[0]), focal, out)
         return self._run(L.CS_MAX, focal, obj_offset, obj, meters, None, out, stream)
 
 
