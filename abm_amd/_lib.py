@@ -90,6 +90,24 @@ class ForageState(C.Structure):
         "novelty", "mode", "overriding_mode", "env_status", "exploited_patch_id", "radius")]
 
 
+class ForageWorldParams(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int32), ("patch_radius", C.c_int32), ("min_units", C.c_int32),
+        ("max_units", C.c_int32), ("pooling_time", C.c_int32), ("regenerate", C.c_int32),
+        ("teleport_exploit", C.c_int32), ("allow_border_overlap", C.c_int32),
+        ("consumption", C.c_double), ("min_quality", C.c_double), ("max_quality", C.c_double),
+    ]
+
+
+class PatchState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "count", "id", "x", "y", "radius", "units", "left", "quality")]
+
+
+WORLD_STEP, WORLD_RESUME, WORLD_FILL = 0, 1, 2                       # ABM_WORLD_* phases
+WORLD_DONE, WORLD_NEED_WORDS, WORLD_TIMEOUT, WORLD_BAD_COUNT = 0, 1, 2, 4   # statuses
+
+
 class CSParams(C.Structure):
     _fields_ = [
         ("n_rays", C.c_int32), ("has_max_proj_size", C.c_int32),
@@ -113,6 +131,8 @@ _SIGS = {
     "abm_build_info": (C.c_char_p, []),
     "abm_mt19937_peek": (C.c_int, [_P, _I32, _P, _P]),
     "abm_mt19937_skip": (C.c_int, [_P, _I32, _P]),
+    "abm_mt19937_peek32": (C.c_int, [_P, _I32, _P, _P]),
+    "abm_mt19937_skip32": (C.c_int, [_P, _I32, _P]),
     "abm_vf_batch_create": (C.c_int, [C.POINTER(VFReplica), _I32, C.POINTER(_P)]),
     "abm_vf_batch_destroy": (C.c_int, [_P]),
     "abm_vf_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
@@ -135,6 +155,11 @@ _SIGS = {
     "abm_forage_step_f64": (C.c_int, [_P, C.POINTER(ForageState), _P, _P, _P, _P, _P, _P]),
     "abm_forage_projection": (C.c_int, [_I32, _D, _D, _I32, _D, _I32, _D, _D, _D, _D,
                                         _P, _P, _I32, _I32, _P, _P]),
+    "abm_forage_world_create": (C.c_int, [_P, C.POINTER(ForageWorldParams), C.POINTER(_P)]),
+    "abm_forage_world_destroy": (C.c_int, [_P]),
+    "abm_forage_world_get_offsets": (C.c_int, [_P, _P]),
+    "abm_forage_world_run": (C.c_int, [_P, _I32, C.POINTER(ForageState), _P, C.POINTER(PatchState),
+                                       _P, _P, _P, _P, _P, _P, _P]),
     "abm_cs_projection": (C.c_int, [C.POINTER(CSParams), _I32, _P, _P, _P, _P, _P, _I32, _P, _P]),
     "abm_ensemble_stats": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32, _D, _D, _P, _P]),
     "abm_log_record": (C.c_int, [C.POINTER(LogBuffers), C.c_int64, C.c_int64, _I32, _P, _P, _P, _P,

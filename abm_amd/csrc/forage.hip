@@ -263,13 +263,28 @@ __device__ inline int soc_bit(const uint8_t* V, int k, int R, double step, doubl
     return V[R - 1 - k];
 }
 
+// Status of a replica between the interaction kernel and the update.
+constexpr int kWorldReady = 3;
+
+// numpy's legacy random_sample from two raw MT19937 outputs (mt19937_next_double).
+__device__ inline double words_to_double(uint32_t w0, uint32_t w1) {
+    const uint32_t a = w0 >> 5, b = w1 >> 6;
+    return ((double)a * 67108864.0 + (double)b) / 9007199254740992.0;
+}
+
+// The random walk's uniforms come either as doubles (uniforms) or as raw
+// 32-bit outputs (words, two per draw, starting after the words the resource
+// loop consumed: u_used[b] on entry).  With status, only replicas marked
+// kWorldReady are updated, and they are marked ABM_WORLD_DONE.
 __global__ void __launch_bounds__(256)
 forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
-                   const double* __restrict__ uniforms, const int64_t* __restrict__ u_off,
-                   const int32_t* __restrict__ u_cnt, int32_t* __restrict__ u_used,
+                   const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
+                   const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
+                   int32_t* __restrict__ u_used, int32_t* __restrict__ status,
                    uint32_t* __restrict__ bits_out, int maxN, int maxR) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) double smem[];
+    if (status && status[blockIdx.x] != kWorldReady) return;   // uniform over the block
     const FGRep P = reps[blockIdx.x];
     const int N = P.n_agents, R = P.n_rays, a0 = P.agent_offset;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -286,9 +301,10 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         S.mode[i] = st.mode[g]; S.over[i] = st.overriding_mode[g];
         S.env[i] = st.env_status[g]; S.pid[i] = st.exploited_patch_id[g];
     }
-    const double* ubuf = uniforms + u_off[blockIdx.x];
+    const double* ubuf = words ? nullptr : uniforms + u_off[blockIdx.x];
+    const uint32_t* wbuf = words ? words + u_off[blockIdx.x] : nullptr;
     const int ucount = u_cnt[blockIdx.x];
-    int used = 0;   // meaningful in thread 0
+    int used = words ? u_used[blockIdx.x] : 0;   // meaningful in thread 0
     const int nl = R / 2;
     __syncthreads();
 
@@ -361,8 +377,14 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             double dvel = 0.0, dth = 0.0;
             const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
             auto walk = [&]() {
-                const double uu = used < ucount ? ubuf[used] : NAN;
-                ++used;
+                double uu = NAN;
+                if (wbuf) {
+                    if (used + 2 <= ucount) uu = words_to_double(wbuf[used], wbuf[used + 1]);
+                    used += 2;
+                } else {
+                    if (used < ucount) uu = ubuf[used];
+                    ++used;
+                }
                 dvel = P.exp_vel_max;
                 dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
                 over = -1; mode = ABM_MODE_EXPLORE;
@@ -407,6 +429,341 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         st.mode[g] = S.mode[i]; st.overriding_mode[g] = S.over[i];
     }
     if (tid == 0 && u_used) u_used[blockIdx.x] = used;
+    if (tid == 0 && status) status[blockIdx.x] = ABM_WORLD_DONE;
+}
+
+// ------------------------------------------------ agent-resource interaction
+constexpr int kMaxPatches = 64;     // patches per replica (one 64-bit membership word)
+constexpr int kMaxRetries = 10000;  // add_new_resource_patch's max_retries (sims.py:335)
+
+// Pending patch placements of a replica, kept between a phase that ran out of
+// words and its resume.
+constexpr int kPendNone = 0, kPendKill = 1, kPendFill = 2;
+
+// Per-replica world constants on the device.
+struct WRep {
+    int32_t n_patches, patch_radius, pooling_time, regenerate, teleport, pad_;
+    int64_t patch_offset;
+    int64_t x_lo, x_hi, y_lo, y_hi, units_lo, units_hi;   // randint bounds [lo, hi)
+    double consumption, q_lo, q_hi;
+};
+
+// Raw MT19937 outputs of one replica; reading past the end sets `out`.
+struct WordCursor {
+    const uint32_t* w;
+    int used, limit;
+    bool out;
+    __device__ uint32_t next() {
+        if (used >= limit) { out = true; return 0u; }
+        return w[used++];
+    }
+};
+
+// RandomState.randint(lo, hi) for a range below 2^32 - 1: numpy draws
+// next_uint32 & mask until the value is <= hi - 1 - lo, mask being the
+// smallest all-ones word covering it (random_bounded_uint64_fill with
+// use_masked; no draw when the range holds one value).
+__device__ inline int64_t np_randint(WordCursor& c, int64_t lo, int64_t hi) {
+    const uint32_t rng = (uint32_t)(hi - 1 - lo);
+    if (rng == 0u) return lo;
+    uint32_t mask = rng;
+    mask |= mask >> 1; mask |= mask >> 2; mask |= mask >> 4; mask |= mask >> 8; mask |= mask >> 16;
+    uint32_t v;
+    do { v = c.next() & mask; } while (v > rng && !c.out);
+    return lo + (int64_t)v;
+}
+
+// RandomState.uniform(lo, hi) = lo + (hi - lo) * random_sample.
+__device__ inline double np_uniform(WordCursor& c, double lo, double hi) {
+#pragma clang fp contract(off)
+    const uint32_t w0 = c.next(), w1 = c.next();
+    return lo + (hi - lo) * words_to_double(w0, w1);
+}
+
+// Python's float x % (2*pi) (and numpy's remainder): fmod, shifted into
+// [0, 2*pi) when negative, +0 for a zero.
+__device__ inline double py_mod_2pi(double x) {
+#pragma clang fp contract(off)
+    double m = fmod(x, kTwoPi);
+    if (m != 0.0) {
+        if (m < 0.0) m = m + kTwoPi;
+    } else {
+        m = 0.0;
+    }
+    return m;
+}
+
+// notify_agent (sims.py:29-42): env status, novelty ring, pool_success, patch id.
+__device__ inline void notify(const abm_forage_state& st, int32_t* pool_success, int tau, int g,
+                              int status, int patch_id) {
+    const int before = st.env_status[g];
+    st.env_status[g] = status;
+    double* nov = st.novelty + (int64_t)g * tau;
+    for (int q = tau - 1; q > 0; --q) nov[q] = nov[q - 1];   // np.roll(novelty, 1)
+    nov[0] = (status - before) > 0 ? 1.0 : 0.0;
+    pool_success[g] = 1;
+    st.exploited_patch_id[g] = patch_id;
+}
+
+// The condition under which the loop notifies an agent (sims.py:816-817, 852).
+__device__ inline bool pooled(int mode, int pool_success, int pooling_time) {
+    return ((mode == ABM_MODE_POOL || mode == ABM_MODE_RELOCATE) && pool_success) || pooling_time == 0;
+}
+
+// pygame.sprite.collide_circle of two patches: integer rect centres,
+// squared distance <= (r1 + r2)^2.
+__device__ inline bool patches_collide(int64_t cx1, int64_t cy1, int64_t r1, int64_t cx2, int64_t cy2,
+                                       int64_t r2) {
+    const int64_t dx = cx1 - cx2, dy = cy1 - cy2, rr = r1 + r2;
+    return dx * dx + dy * dy <= rr * rr;
+}
+
+struct FreshPatch {
+    int32_t id;
+    int64_t x, y;
+    double units, quality;
+};
+
+// Place the replica's pending patches (regenerations of the patches killed
+// this step, in kill order, or create_resources' new patches) from the word
+// buffer, then compact the list: live patches in order, new ones appended.
+// One thread.  Commits nothing unless every placement fits in the buffer.
+__device__ int place_patches(const WRep& W, abm_patch_state ps, int b, int kind, int n_pending,
+                             const int32_t* __restrict__ pend_id, int32_t* __restrict__ kill_rank,
+                             WordCursor& cur, FreshPatch* fresh) {
+    const int64_t po = W.patch_offset;
+    const int count = ps.count[b];
+    int n_new = 0;
+    if (kind == kPendKill) n_new = W.regenerate ? n_pending : 0;
+    else if (kind == kPendFill) n_new = W.n_patches - count;
+    const int64_t r = W.patch_radius;
+    for (int k = 0; k < n_new; ++k) {
+        int32_t id;
+        if (kind == kPendKill) {
+            id = pend_id[po + k];                 // force_id: the killed patch's id
+        } else {
+            bool any = false;                     // max(ids) + 1, or 1 for an empty list
+            int32_t mx = 0;
+            for (int t = 0; t < count; ++t) {
+                const int32_t v = ps.id[po + t];
+                if (!any || v > mx) mx = v;
+                any = true;
+            }
+            for (int f = 0; f < k; ++f) {
+                if (!any || fresh[f].id > mx) mx = fresh[f].id;
+                any = true;
+            }
+            id = (any ? mx : 0) + 1;
+        }
+        int retries = 0;
+        for (;;) {
+            if (retries > kMaxRetries) return ABM_WORLD_TIMEOUT;
+            const int64_t x = np_randint(cur, W.x_lo, W.x_hi);
+            const int64_t y = np_randint(cur, W.y_lo, W.y_hi);
+            const int64_t units = np_randint(cur, W.units_lo, W.units_hi);
+            const double q = np_uniform(cur, W.q_lo, W.q_hi);
+            if (cur.out) return ABM_WORLD_NEED_WORDS;
+            const int64_t cx = x + r, cy = y + r;
+            bool ok = true;
+            // the Group at the time: live patches, those killed later this
+            // step, and the ones placed before this one
+            for (int t = 0; t < count && ok; ++t) {
+                const int64_t s = po + t;
+                if (kill_rank[s] >= 0 && kill_rank[s] <= k) continue;
+                const double pr = ps.radius[s];
+                const int64_t pcx = (int64_t)trunc(ps.x[s] + pr), pcy = (int64_t)trunc(ps.y[s] + pr);
+                if (patches_collide(pcx, pcy, (int64_t)pr, cx, cy, r)) ok = false;
+            }
+            for (int f = 0; f < k && ok; ++f)
+                if (patches_collide(fresh[f].x + r, fresh[f].y + r, r, cx, cy, r)) ok = false;
+            ++retries;
+            if (ok) {
+                fresh[k].id = id; fresh[k].x = x; fresh[k].y = y;
+                fresh[k].units = (double)units; fresh[k].quality = q;
+                break;
+            }
+        }
+    }
+    // commit: drop the killed patches, keep the order, append the new ones
+    int wpos = 0;
+    for (int t = 0; t < count; ++t) {
+        const int64_t s = po + t;
+        if (kill_rank[s] >= 0) { kill_rank[s] = -1; continue; }
+        if (wpos != t) {
+            const int64_t d = po + wpos;
+            ps.id[d] = ps.id[s]; ps.x[d] = ps.x[s]; ps.y[d] = ps.y[s]; ps.radius[d] = ps.radius[s];
+            ps.units[d] = ps.units[s]; ps.left[d] = ps.left[s]; ps.quality[d] = ps.quality[s];
+        }
+        ++wpos;
+    }
+    for (int k = 0; k < n_new; ++k) {
+        const int64_t d = po + wpos + k;
+        ps.id[d] = fresh[k].id; ps.x[d] = (double)fresh[k].x; ps.y[d] = (double)fresh[k].y;
+        ps.radius[d] = (double)r; ps.units[d] = fresh[k].units; ps.left[d] = fresh[k].units;
+        ps.quality[d] = fresh[k].quality;
+    }
+    ps.count[b] = wpos + n_new;
+    return ABM_WORLD_DONE;
+}
+
+// One replica per block.  STEP: agent-resource interaction (sims.py:789-855)
+// then the placement of regenerated patches; FILL: create_resources; RESUME:
+// the placements a previous phase left pending.  Membership is computed for
+// every (patch, agent) pair in parallel from the poses at the start of the
+// step; the loop over patches and their agents, whose depletion, notices and
+// kills depend on the order, runs in id order in one thread.
+__global__ void __launch_bounds__(256)
+forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wreps, int phase,
+                    abm_forage_state st, int32_t* __restrict__ pool_success, abm_patch_state ps,
+                    int32_t* __restrict__ kill_rank, int32_t* __restrict__ pend,
+                    int32_t* __restrict__ pend_id, const uint32_t* __restrict__ words,
+                    const int64_t* __restrict__ w_off, const int32_t* __restrict__ w_cnt,
+                    int32_t* __restrict__ w_used, int32_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) uint64_t member[];   // [patch][64-agent chunk]
+    __shared__ FreshPatch fresh[kMaxPatches];
+    const int b = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    const FGRep P = reps[b];
+    const WRep W = wreps[b];
+    const int N = P.n_agents, a0 = P.agent_offset, C = (N + 63) / 64;
+    const int64_t po = W.patch_offset;
+    int kind = pend[2 * b], n_pending = pend[2 * b + 1];
+    const int n_live = ps.count[b];
+    if (n_live < 0 || n_live > W.n_patches) {
+        if (tid == 0) { w_used[b] = 0; status[b] = ABM_WORLD_BAD_COUNT; }
+        return;
+    }
+    if (phase == ABM_WORLD_RESUME) {
+        if (status[b] != ABM_WORLD_NEED_WORDS || kind == kPendNone) {
+            if (tid == 0) w_used[b] = 0;
+            return;
+        }
+    } else if (kind != kPendNone) {
+        // placements of an earlier phase still wait for words: resume first
+        if (tid == 0) { w_used[b] = 0; status[b] = ABM_WORLD_NEED_WORDS; }
+        return;
+    }
+    if (phase == ABM_WORLD_STEP) {
+        const int np_ = n_live;
+        // ---- membership: supcalc.distance(resc, agent) < resc.radius
+        const int total = np_ * C * 64;
+        for (int base = tid & ~63; base < total; base += nthr) {
+            const int idx = base + (tid & 63);
+            const int p = idx / (C * 64), j = idx - p * (C * 64);
+            bool in = false;
+            if (j < N) {
+                const int64_t s = po + p;
+                const double pr = ps.radius[s];
+                const double pcx = ps.x[s] + pr, pcy = ps.y[s] + pr;
+                const int g = a0 + j;
+                const double ra = (double)st.radius[g];
+                const double dx = (st.x[g] + ra) - pcx, dy = (st.y[g] + ra) - pcy;
+                in = norm2(dx, dy) < pr;
+            }
+            const uint64_t m = __ballot(in);
+            if ((tid & 63) == 0) member[p * C + (j >> 6)] = m;
+        }
+        __syncthreads();
+        // ---- the patches' agent groups in order (sims.py:803-845)
+        if (tid == 0) {
+            int n_kill = 0;
+            for (int p = 0; p < np_; ++p) {
+                const int64_t s = po + p;
+                const double pr = ps.radius[s];
+                const double pcx = ps.x[s] + pr, pcy = ps.y[s] + pr;   // resc.center
+                const int32_t pid = ps.id[s];
+                bool destroy = false;
+                for (int c = 0; c < C; ++c) {
+                    uint64_t m = member[p * C + c];
+                    while (m) {
+                        const int g = a0 + c * 64 + __builtin_ctzll(m);
+                        m &= m - 1;
+                        const double ra = (double)st.radius[g];
+                        // bias_agent_towards_res_center (sims.py:544-552)
+                        const double x1 = st.x[g] + ra, y1 = st.y[g] + ra;
+                        const double th = st.theta[g];
+                        const double cl = py_mod_2pi(atan2(pcy - y1, pcx - x1) + th);
+                        st.theta[g] = th + (cl - kPi) * 0.02;
+                        if (destroy) {
+                            notify(st, pool_success, P.tau, g, -1, -1);
+                            continue;
+                        }
+                        const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
+                        if (pooled(mode, pool_success[g], W.pooling_time)) {
+                            notify(st, pool_success, P.tau, g, 1, pid);
+                            if (W.teleport) {
+                                st.x[g] = (ps.x[s] + pr) - ra;
+                                st.y[g] = (ps.y[s] + pr) - ra;
+                            }
+                        }
+                        if (mode == ABM_MODE_EXPLOIT) {
+                            // Rescource.deplete (rescource.py:118-133)
+                            double units = W.consumption;
+                            const double q = ps.quality[s];
+                            if (units > q) units = q;
+                            double left = ps.left[s], dep;
+                            if (left >= units) { left = left - units; dep = units; }
+                            else { dep = left; left = 0.0; }
+                            ps.left[s] = left;
+                            destroy = !(left > 0.0);
+                            const double cr = st.collected_r[g];
+                            st.collected_r_before[g] = cr;
+                            st.collected_r[g] = cr + dep;
+                            if (destroy) {
+                                // every agent of the group hears of it
+                                for (int c2 = 0; c2 < C; ++c2) {
+                                    uint64_t m2 = member[p * C + c2];
+                                    while (m2) {
+                                        const int g2 = a0 + c2 * 64 + __builtin_ctzll(m2);
+                                        m2 &= m2 - 1;
+                                        notify(st, pool_success, P.tau, g2, -1, -1);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                if (destroy) {
+                    kill_rank[s] = n_kill;
+                    pend_id[po + n_kill] = pid;
+                    ++n_kill;
+                }
+            }
+            kind = n_kill ? kPendKill : kPendNone;
+            n_pending = n_kill;
+        }
+        // ---- agents on no patch (sims.py:847-855)
+        for (int j = tid; j < N; j += nthr) {
+            bool on = false;
+            for (int p = 0; p < np_; ++p) on |= ((member[p * C + (j >> 6)] >> (j & 63)) & 1ull) != 0;
+            if (on) continue;
+            const int g = a0 + j;
+            const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
+            if (pooled(mode, pool_success[g], W.pooling_time) || mode == ABM_MODE_EXPLOIT)
+                notify(st, pool_success, P.tau, g, -1, -1);
+        }
+    } else if (phase == ABM_WORLD_FILL) {
+        kind = kPendFill;
+        n_pending = 0;
+    }
+    if (tid != 0) return;
+    // ---- placements: kept words for the random walk of the update that follows
+    const bool update_follows = kind != kPendFill;
+    const int reserve = update_follows ? 2 * N : 0;
+    const int limit = w_cnt ? max(0, w_cnt[b] - reserve) : 0;
+    WordCursor cur{words ? words + w_off[b] : nullptr, 0, words ? limit : 0, false};
+    int rc = ABM_WORLD_DONE;
+    if (kind != kPendNone) rc = place_patches(W, ps, b, kind, n_pending, pend_id, kill_rank, cur, fresh);
+    if (rc == ABM_WORLD_DONE) {
+        pend[2 * b] = kPendNone; pend[2 * b + 1] = 0;
+        w_used[b] = cur.used;
+        status[b] = update_follows ? kWorldReady : ABM_WORLD_DONE;
+    } else {
+        // nothing placed and nothing drawn; the kill ranks stay for the resume
+        pend[2 * b] = kind; pend[2 * b + 1] = n_pending;
+        w_used[b] = rc == ABM_WORLD_TIMEOUT ? cur.used : 0;
+        status[b] = rc;
+    }
 }
 
 // Agent.projection_field of one focal agent: obstacles [0, n_social) are the
@@ -448,6 +805,44 @@ forage_projection_kernel(int R, double step, double fov0, double fov1, int exclu
 }  // namespace abm
 
 using namespace abm;
+
+namespace {
+
+int check_state(const abm_forage_state* st, const char* who) {
+    const void* req[] = {st->x, st->y, st->theta, st->v, st->w, st->u, st->I_priv, st->collected_r,
+                         st->collected_r_before, st->novelty, st->mode, st->overriding_mode,
+                         st->env_status, st->exploited_patch_id, st->radius};
+    for (const void* p : req)
+        if (!p) return set_error(ABM_ERR_INVALID, "%s: every state array is required", who);
+    return ABM_OK;
+}
+
+// Agent.update for every replica (all, or those a status array marks ready).
+int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
+                       const uint32_t* words, const int64_t* offset, const int32_t* count,
+                       int32_t* used, int32_t* status, uint32_t* field_bits, void* stream) {
+    const int nthr = b->max_agents > 128 ? 256 : (b->max_agents > 64 ? 128 : 64);
+    const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
+    int dev_max = 0;
+    if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                 b->device), "hipDeviceGetAttribute"))
+        return rc;
+    if (lds > (size_t)dev_max)
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_step_f64: %d agents x %d rays need %zu B of LDS",
+                         b->max_agents, b->max_rays, lds);
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
+                       (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
+                       field_bits, b->max_agents, b->max_rays);
+    return check_hip(hipGetLastError(), "forage_step_kernel launch");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -527,36 +922,132 @@ int abm_forage_batch_get_offsets(const abm_forage_batch* b, int32_t* agent_offse
     return ABM_OK;
 }
 
+
 int abm_forage_step_f64(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
                         const int64_t* uniform_offset, const int32_t* uniform_count,
                         int32_t* uniforms_used, uint32_t* field_bits, void* stream) {
     if (!b || !st) return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: NULL batch or state");
-    const void* req[] = {st->x, st->y, st->theta, st->v, st->w, st->u, st->I_priv, st->collected_r,
-                         st->collected_r_before, st->novelty, st->mode, st->overriding_mode,
-                         st->env_status, st->exploited_patch_id, st->radius};
-    for (const void* p : req)
-        if (!p) return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: every state array is required");
+    if (int rc = check_state(st, "abm_forage_step_f64")) return rc;
     if (!uniforms || !uniform_offset || !uniform_count)
         return set_error(ABM_ERR_INVALID, "abm_forage_step_f64: the uniform buffer is required");
-    const int nthr = b->max_agents > 128 ? 256 : (b->max_agents > 64 ? 128 : 64);
-    const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
-    int dev_max = 0;
-    if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                                 b->device), "hipDeviceGetAttribute"))
+    return launch_forage_step(b, st, uniforms, nullptr, uniform_offset, uniform_count, uniforms_used,
+                              nullptr, field_bits, stream);
+}
+
+int abm_forage_world_create(abm_forage_batch* b, const abm_forage_world_params* params,
+                            abm_forage_world** out) {
+    if (!out) return set_error(ABM_ERR_INVALID, "abm_forage_world_create: out is NULL");
+    *out = nullptr;
+    if (!b || !params) return set_error(ABM_ERR_INVALID, "abm_forage_world_create: NULL batch or params");
+    const int n = b->n_replicas;
+    std::vector<FGRep> reps(n);
+    if (int rc = check_hip(hipMemcpy(reps.data(), b->reps_dev, sizeof(FGRep) * n, hipMemcpyDeviceToHost),
+                           "download forage replicas"))
         return rc;
-    if (lds > (size_t)dev_max)
-        return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_step_f64: %d agents x %d rays need %zu B of LDS",
-                         b->max_agents, b->max_rays, lds);
-    if (lds > 65536) {
-        if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                               "hipFuncSetAttribute"))
-            return rc;
+    std::vector<WRep> host(n);
+    int64_t total = 0;
+    size_t lds = 8;
+    const int64_t kRange = 0xFFFFFFFEll;   // randint ranges up to 2^32 - 2 values
+    for (int i = 0; i < n; ++i) {
+        const abm_forage_world_params& q = params[i];
+        WRep& w = host[i];
+        std::memset(&w, 0, sizeof(w));
+        if (q.n_patches < 0 || q.n_patches > kMaxPatches)
+            return set_error(ABM_ERR_INVALID, "forage world %d: n_patches=%d out of range [0, %d]", i,
+                             q.n_patches, kMaxPatches);
+        if (q.patch_radius < 0)
+            return set_error(ABM_ERR_INVALID, "forage world %d: negative patch radius", i);
+        w.n_patches = q.n_patches; w.patch_radius = q.patch_radius; w.pooling_time = q.pooling_time;
+        w.regenerate = q.regenerate ? 1 : 0; w.teleport = q.teleport_exploit ? 1 : 0;
+        w.patch_offset = total;
+        // the bounds numpy sees: int() of the Python expressions (sims.py:349-356)
+        const double r = (double)q.patch_radius, pad = reps[i].pad;
+        const double W = reps[i].width, H = reps[i].height;
+        if (q.allow_border_overlap) {
+            w.x_lo = (int64_t)trunc(pad - r); w.x_hi = (int64_t)trunc(W + pad - r);
+            w.y_lo = (int64_t)trunc(pad - r); w.y_hi = (int64_t)trunc(H + pad - r);
+        } else {
+            w.x_lo = (int64_t)trunc(pad); w.x_hi = (int64_t)trunc(W + pad - 2.0 * r);
+            w.y_lo = (int64_t)trunc(pad); w.y_hi = (int64_t)trunc(H + pad - 2.0 * r);
+        }
+        w.units_lo = q.min_units; w.units_hi = q.max_units;
+        const int64_t lo[] = {w.x_lo, w.y_lo, w.units_lo}, hi[] = {w.x_hi, w.y_hi, w.units_hi};
+        for (int k = 0; k < 3; ++k)
+            if (q.n_patches > 0 && (hi[k] <= lo[k] || hi[k] - 1 - lo[k] > kRange))
+                return set_error(ABM_ERR_INVALID, "forage world %d: randint(%lld, %lld) has no values "
+                                 "(numpy raises low >= high) or too many", i, (long long)lo[k],
+                                 (long long)hi[k]);
+        w.consumption = q.consumption; w.q_lo = q.min_quality; w.q_hi = q.max_quality;
+        if (!std::isfinite(q.max_quality - q.min_quality))
+            return set_error(ABM_ERR_INVALID, "forage world %d: quality range is not finite", i);
+        total += q.n_patches;
+        lds = std::max(lds, sizeof(uint64_t) * (size_t)q.n_patches * (size_t)((reps[i].n_agents + 63) / 64));
     }
-    hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
-                       (const FGRep*)b->reps_dev, *st, uniforms, uniform_offset, uniform_count,
-                       uniforms_used, field_bits, b->max_agents, b->max_rays);
-    return check_hip(hipGetLastError(), "forage_step_kernel launch");
+    abm_forage_world* wd = new abm_forage_world();
+    wd->batch = b; wd->n_replicas = n; wd->patch_total = total; wd->member_lds = lds;
+    const size_t slots = (size_t)std::max<int64_t>(total, 1);
+    int rc = check_hip(hipMalloc(&wd->wreps_dev, sizeof(WRep) * n), "hipMalloc(world replicas)");
+    if (!rc) rc = check_hip(hipMalloc(&wd->kill_rank, sizeof(int32_t) * slots), "hipMalloc(kill ranks)");
+    if (!rc) rc = check_hip(hipMalloc(&wd->pend, sizeof(int32_t) * 2 * n), "hipMalloc(pending)");
+    if (!rc) rc = check_hip(hipMalloc(&wd->pend_id, sizeof(int32_t) * slots), "hipMalloc(pending ids)");
+    if (!rc) rc = check_hip(hipMemcpy(wd->wreps_dev, host.data(), sizeof(WRep) * n, hipMemcpyHostToDevice),
+                           "upload world replicas");
+    if (!rc) rc = check_hip(hipMemset(wd->kill_rank, 0xff, sizeof(int32_t) * slots), "clear kill ranks");
+    if (!rc) rc = check_hip(hipMemset(wd->pend, 0, sizeof(int32_t) * 2 * n), "clear pending");
+    if (!rc) rc = check_hip(hipDeviceSynchronize(), "world create");
+    if (rc) { abm_forage_world_destroy(wd); return rc; }
+    wd->patch_offset_host = new int64_t[n + 1];
+    for (int i = 0; i < n; ++i) wd->patch_offset_host[i] = host[i].patch_offset;
+    wd->patch_offset_host[n] = total;
+    *out = wd;
+    return ABM_OK;
+}
+
+int abm_forage_world_destroy(abm_forage_world* w) {
+    if (!w) return ABM_OK;
+    if (w->wreps_dev) (void)hipFree(w->wreps_dev);
+    if (w->kill_rank) (void)hipFree(w->kill_rank);
+    if (w->pend) (void)hipFree(w->pend);
+    if (w->pend_id) (void)hipFree(w->pend_id);
+    delete[] w->patch_offset_host;
+    delete w;
+    return ABM_OK;
+}
+
+int abm_forage_world_get_offsets(const abm_forage_world* w, int64_t* patch_offset) {
+    if (!w || !patch_offset) return set_error(ABM_ERR_INVALID, "abm_forage_world_get_offsets: NULL argument");
+    std::copy(w->patch_offset_host, w->patch_offset_host + w->n_replicas + 1, patch_offset);
+    return ABM_OK;
+}
+
+int abm_forage_world_run(abm_forage_world* w, int32_t phase, const abm_forage_state* st,
+                         int32_t* pool_success, const abm_patch_state* ps, const uint32_t* words,
+                         const int64_t* word_offset, const int32_t* word_count, int32_t* words_used,
+                         int32_t* status, uint32_t* field_bits, void* stream) {
+    if (!w || !st || !ps) return set_error(ABM_ERR_INVALID, "abm_forage_world_run: NULL world, state or patches");
+    if (phase != ABM_WORLD_STEP && phase != ABM_WORLD_RESUME && phase != ABM_WORLD_FILL)
+        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: unknown phase %d", phase);
+    if (int rc = check_state(st, "abm_forage_world_run")) return rc;
+    if (!pool_success || !words_used || !status)
+        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: pool_success, words_used and status are required");
+    if (w->patch_total > 0 && (!ps->count || !ps->id || !ps->x || !ps->y || !ps->radius || !ps->units ||
+                               !ps->left || !ps->quality))
+        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: every patch array is required");
+    if (!ps->count) return set_error(ABM_ERR_INVALID, "abm_forage_world_run: patch count array is required");
+    if (!words || !word_offset || !word_count)
+        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: the word buffer is required");
+    abm_forage_batch* b = w->batch;
+    const size_t lds = w->member_lds;
+    if (lds > 65536) return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_world_run: %zu B of membership masks", lds);
+    hipLaunchKernelGGL(forage_world_kernel, dim3(b->n_replicas), dim3(256), lds, (hipStream_t)stream,
+                       (const FGRep*)b->reps_dev, (const WRep*)w->wreps_dev, (int)phase, *st, pool_success,
+                       *ps, w->kill_rank, w->pend, w->pend_id, words, word_offset, word_count, words_used,
+                       status);
+    if (int rc = check_hip(hipGetLastError(), "forage_world_kernel launch")) return rc;
+    if (phase == ABM_WORLD_FILL) return ABM_OK;
+    // the update reads its walk's words after those the placements used
+    return launch_forage_step(b, st, nullptr, words, word_offset, word_count, words_used, status,
+                              field_bits, stream);
 }
 
 int abm_forage_projection(int32_t R, double fov0, double fov1, int32_t visual_exclusion,

@@ -17,3 +17,17 @@ struct abm_forage_batch {
     int32_t* agent_offset_host = nullptr;
     int64_t* field_offset_host = nullptr;
 };
+
+// The resource patches' per-replica constants and pending-placement scratch
+// behind the opaque abm_forage_world handle.
+struct abm_forage_world {
+    abm_forage_batch* batch = nullptr;
+    int32_t n_replicas = 0;
+    int64_t patch_total = 0;
+    size_t member_lds = 0;           // bytes of the membership masks per block
+    void* wreps_dev = nullptr;
+    int32_t* kill_rank = nullptr;    // [patch_total], -1 = not killed
+    int32_t* pend = nullptr;         // [2 * n_replicas]: kind, count
+    int32_t* pend_id = nullptr;      // [patch_total]
+    int64_t* patch_offset_host = nullptr;
+};

@@ -89,4 +89,35 @@ int abm_mt19937_skip(abm_mt19937_state* const* states, int32_t n_states, const i
     return ABM_OK;
 }
 
+int abm_mt19937_peek32(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts,
+                       uint32_t* out) {
+    if (int rc = check_args(states, n_states, counts, "abm_mt19937_peek32")) return rc;
+    int64_t total = 0;
+    for (int32_t b = 0; b < n_states; ++b) {
+        if (!states[b] || counts[b] < 0 || states[b]->pos < 0 || states[b]->pos > kN)
+            return set_error(ABM_ERR_INVALID, "abm_mt19937_peek32: state %d is null, has a negative "
+                             "count or a position outside [0, 624]", b);
+        total += counts[b];
+    }
+    if (total > 0 && !out) return set_error(ABM_ERR_INVALID, "abm_mt19937_peek32: null output");
+    abm_mt19937_state copy;
+    for (int32_t b = 0; b < n_states; ++b) {
+        if (counts[b] == 0) continue;
+        std::memcpy(&copy, states[b], sizeof(copy));
+        for (int64_t i = 0; i < counts[b]; ++i) *out++ = next32(&copy);
+    }
+    return ABM_OK;
+}
+
+int abm_mt19937_skip32(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts) {
+    if (int rc = check_args(states, n_states, counts, "abm_mt19937_skip32")) return rc;
+    for (int32_t b = 0; b < n_states; ++b)
+        if (!states[b] || counts[b] < 0 || states[b]->pos < 0 || states[b]->pos > kN)
+            return set_error(ABM_ERR_INVALID, "abm_mt19937_skip32: state %d is null, has a negative "
+                             "count or a position outside [0, 624]", b);
+    for (int32_t b = 0; b < n_states; ++b)
+        for (int64_t i = 0; i < counts[b]; ++i) (void)next32(states[b]);
+    return ABM_OK;
+}
+
 }  // extern "C"

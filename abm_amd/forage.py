@@ -238,6 +238,172 @@ class ForageSwarm:
         return unpack_bits(w.reshape(cfg.n_agents, words), cfg.n_rays)
 
 
+# ------------------------------------------------ resources on the device
+@dataclass
+class WorldConfig:
+    """Resource patches of one foraging replica: the Simulation kwargs of the
+    resource loop (sims.py:60-68, 160-177) with their defaults, except
+    ``pooling_time``, which defaults to 0 as in the reference's .env (with a
+    positive pooling time the reference reads ``pool_success`` before any
+    notify has set it; here it starts at 0)."""
+    N_resc: int = 10
+    patch_radius: int = 30
+    min_resc_perpatch: int = 200
+    max_resc_perpatch: int = 1000
+    min_resc_quality: float = 0.1
+    max_resc_quality: float = 1.0
+    pooling_time: int = 0
+    regenerate_patches: bool = True
+    agent_consumption: float = 1.0
+    teleport_exploit: bool = True
+    allow_border_patch_overlap: bool = False
+
+    def to_c(self) -> L.ForageWorldParams:
+        max_q = self.min_resc_quality if self.max_resc_quality < 0 else self.max_resc_quality
+        max_u = self.min_resc_perpatch + 1 if self.max_resc_perpatch < 0 else self.max_resc_perpatch
+        return L.ForageWorldParams(
+            n_patches=int(self.N_resc), patch_radius=int(self.patch_radius),
+            min_units=int(self.min_resc_perpatch), max_units=int(max_u),
+            pooling_time=int(self.pooling_time), regenerate=int(bool(self.regenerate_patches)),
+            teleport_exploit=int(bool(self.teleport_exploit)),
+            allow_border_overlap=int(bool(self.allow_border_patch_overlap)),
+            consumption=float(self.agent_consumption), min_quality=float(self.min_resc_quality),
+            max_quality=float(max_q))
+
+
+_PATCH_F64 = ("x", "y", "radius", "units", "left", "quality")
+TIMEOUT_MESSAGE = "Reached timeout while trying to create resources without overlap!"
+
+
+class ResourceWorld:
+    """Resource patches of every replica of a ``ForageSwarm`` and the whole
+    loop body of ``Simulation.start`` around them (sims.py:789-861, agent
+    collisions off) on the device: ``create_resources`` places the initial
+    patches (sims.py:538-542), ``step`` runs the agent-resource interaction,
+    the regeneration of depleted patches and ``Agent.update`` for all
+    replicas.
+
+    Random numbers: each replica's numpy generator is peeked as raw 32-bit
+    words (patch placement draws randint and uniform from them, the random
+    walk two per uniform) and advanced afterwards by exactly the words used.
+    A replica whose placements need more than ``regen_words`` words is
+    resumed with a larger buffer; nothing is drawn twice."""
+
+    def __init__(self, swarm: ForageSwarm, configs: Sequence[WorldConfig], regen_words: int = 32):
+        self.swarm, self.lib = swarm, swarm.lib
+        self.configs = list(configs)
+        nb = len(swarm.configs)
+        if len(self.configs) != nb:
+            raise ValueError("give one WorldConfig per replica")
+        arr = (L.ForageWorldParams * nb)(*[c.to_c() for c in self.configs])
+        h = C.c_void_p()
+        with torch.cuda.device(swarm.device):
+            L.check(self.lib.abm_forage_world_create(swarm.handle, arr, C.byref(h)))
+        self.handle = h
+        self.patch_offset = np.zeros(nb + 1, dtype=np.int64)
+        L.check(self.lib.abm_forage_world_get_offsets(h, self.patch_offset.ctypes.data_as(C.c_void_p)))
+        dev = swarm.device
+        slots = max(1, int(self.patch_offset[-1]))
+        self.p = {k: torch.zeros(slots, dtype=torch.float64, device=dev) for k in _PATCH_F64}
+        self.p["id"] = torch.zeros(slots, dtype=torch.int32, device=dev)
+        self.p["count"] = torch.zeros(nb, dtype=torch.int32, device=dev)
+        self.pool_success = torch.zeros(int(swarm.info.n_agents_total), dtype=torch.int32, device=dev)
+        self._ret = torch.zeros((2, nb), dtype=torch.int32, device=dev)   # used, status
+        self._woff = torch.zeros(nb, dtype=torch.int64, device=dev)
+        self._wcnt = torch.zeros(nb, dtype=torch.int32, device=dev)
+        self._words_host = torch.zeros(0, dtype=torch.int32).pin_memory()
+        self._words = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.regen_words = int(regen_words)
+        self.resumes = 0      # phases resumed for want of words
+        self._n = np.diff(swarm.agent_offset).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.abm_forage_world_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- patches
+    def patches(self, b):
+        """Replica b's live patches in Group order, as a dict of numpy arrays."""
+        lo = int(self.patch_offset[b])
+        n = int(self.p["count"][b].item())
+        return {k: self.p[k][lo:lo + n].cpu().numpy().copy() for k in ("id",) + _PATCH_F64}
+
+    def set_patches(self, b, id, x, y, radius, units, left, quality):
+        """Overwrite replica b's patch list (Group order)."""
+        lo, cap = int(self.patch_offset[b]), int(self.patch_offset[b + 1] - self.patch_offset[b])
+        n = len(id)
+        if n > cap:
+            raise L.AbmError(L.ABM_ERR_INVALID, f"replica {b} holds at most {cap} patches")
+        vals = dict(id=id, x=x, y=y, radius=radius, units=units, left=left, quality=quality)
+        for k, v in vals.items():
+            t = self.p[k]
+            t[lo:lo + n] = torch.as_tensor(np.asarray(v, dtype=np.int32 if k == "id" else np.float64))
+        self.p["count"][b] = n
+
+    # ---------------------------------------------------------------- phases
+    def create_resources(self, stream=None):
+        """create_resources (sims.py:538-542): fill every replica's list."""
+        self._run(L.WORLD_FILL, False, stream)
+
+    def step(self, fields=False, stream=None):
+        """One loop body for every replica: interaction, regeneration, update."""
+        self._run(L.WORLD_STEP, fields, stream)
+
+    def _run(self, phase, fields, stream):
+        sw = self.swarm
+        nb = len(sw.configs)
+        reserve = np.zeros(nb, dtype=np.int64) if phase == L.WORLD_FILL else 2 * self._n
+        active = np.ones(nb, dtype=bool)
+        budget, ph = self.regen_words, phase
+        st = L.ForageState(**{k: sw.t[k].data_ptr() for k in _F64 + ("novelty",) + _I32})
+        ps = L.PatchState(**{k: self.p[k].data_ptr() for k in ("count", "id") + _PATCH_F64})
+        fb = L.ptr(sw.fields) if fields else None
+        addrs = sw._state_addrs()
+        with torch.cuda.device(sw.device):
+            s = stream if stream is not None else torch.cuda.current_stream()
+            with torch.cuda.stream(s):
+                while True:
+                    cnt = np.where(active, reserve + budget, 0).astype(np.int64)
+                    off = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
+                    total = int(cnt.sum())
+                    if self._words_host.numel() < total:
+                        self._words_host = torch.zeros(total, dtype=torch.int32).pin_memory()
+                    if self._words.numel() < total:
+                        self._words = torch.zeros(total, dtype=torch.int32, device=sw.device)
+                    L.check(self.lib.abm_mt19937_peek32(addrs.ctypes.data_as(C.c_void_p), nb,
+                                                        cnt.ctypes.data_as(C.c_void_p),
+                                                        self._words_host.data_ptr()))
+                    self._words[:total].copy_(self._words_host[:total], non_blocking=True)
+                    self._woff.copy_(torch.from_numpy(off))
+                    self._wcnt.copy_(torch.from_numpy(cnt.astype(np.int32)))
+                    L.check(self.lib.abm_forage_world_run(
+                        self.handle, ph, C.byref(st), L.ptr(self.pool_success), C.byref(ps),
+                        L.ptr(self._words), L.ptr(self._woff), L.ptr(self._wcnt), L.ptr(self._ret[0]),
+                        L.ptr(self._ret[1]), fb, L.stream_ptr(s)))
+                    used, status = self._ret.cpu().numpy()
+                    L.check(self.lib.abm_mt19937_skip32(addrs.ctypes.data_as(C.c_void_p), nb,
+                                                        used.astype(np.int64).ctypes.data_as(C.c_void_p)))
+                    if np.any(status[active] == L.WORLD_TIMEOUT):
+                        raise Exception(TIMEOUT_MESSAGE)
+                    if np.any(status == L.WORLD_BAD_COUNT):
+                        raise L.AbmError(L.ABM_ERR_INVALID, "a patch count is outside [0, N_resc]")
+                    need = status == L.WORLD_NEED_WORDS
+                    if not need.any():
+                        return
+                    budget = max(16 * budget, 64)
+                    if budget > (1 << 24):
+                        raise L.AbmError(L.ABM_ERR_UNSUPPORTED, "patch placement needs more than 2^24 words")
+                    ph, active = L.WORLD_RESUME, need
+                    self.resumes += 1
+
+
 # ------------------------------------------------------- reference API
 def projection_field(agent, obstacles, keep_distance_info=False, non_expl_agents=None, fov=None):
     """Agent.projection_field (agent.py:457-597) on the GPU.

@@ -20,6 +20,10 @@
  *       -> Simulation.start: self.agents.update(self.agents)
  *          (abm/simulation/sims.py:861), i.e. Agent.update in id order
  *          (abm/agent/agent.py:212-283)
+ *   abm_forage_world_run
+ *       -> the whole loop body of Simulation.start with agent collisions off:
+ *          agent-resource interaction and patch regeneration
+ *          (sims.py:789-858, 321-368), then the update above
  * The probe entry points replace single functions:
  *   abm_vf_projection_rows   -> vf_supcalc.projection_field
  *                               (abm/projects/visual_flocking/vf_agent/vf_supcalc.py:20-138)
@@ -343,6 +347,80 @@ int abm_forage_projection(int32_t n_rays, double fov0, double fov1, int32_t visu
                           const double* ox, const double* oy, int32_t n_social,
                           int32_t n_nonsocial, double* out_field, void* stream);
 
+/* ------------------------------------------- agent-resource interaction */
+/* The resource patches of every foraging replica and the loop body around
+ * Agent.update that uses them (abm/simulation/sims.py:789-861 with agent-agent
+ * collisions off): patch membership of agent centres (groupcollide +
+ * refine_ar_overlap_group, sims.py:45-56, 790-799), bias towards the patch
+ * centre (bias_agent_towards_res_center, sims.py:544-552), notify_agent
+ * (sims.py:29-42), teleport, Rescource.deplete (abm/environment/rescource.py:
+ * 118-133), kill_resource and its regeneration (add_new_resource_patch,
+ * sims.py:321-368) and the off-patch notifications (sims.py:847-855).
+ * Per-replica parameters (Simulation kwargs, sims.py:60-68, 160-177). */
+typedef struct abm_forage_world_params {
+    int32_t n_patches;            /* N_RESOURCES: patches created, the list's capacity (<= 64) */
+    int32_t patch_radius;         /* RADIUS_RESOURCE                                  */
+    int32_t min_units, max_units; /* np.random.randint(min_units, max_units), max already
+                                     fixed up as Simulation does when negative          */
+    int32_t pooling_time;         /* POOLING_TIME                                     */
+    int32_t regenerate;           /* REGENERATE_PATCHES                               */
+    int32_t teleport_exploit;     /* TELEPORT_TO_MIDDLE                               */
+    int32_t allow_border_overlap; /* ALLOW_BORDER_PATCH_OVERLAP                       */
+    double  consumption;          /* AGENT_CONSUMPTION                                */
+    double  min_quality, max_quality;  /* np.random.uniform bounds (max fixed up)     */
+} abm_forage_world_params;
+
+/* Patches of all replicas, in each replica's Group order.  Replica b owns the
+ * slots [patch_offset[b], patch_offset[b] + n_patches_b) of every array, of
+ * which the first count[b] are live. */
+typedef struct abm_patch_state {
+    int32_t* count;               /* [n_replicas] */
+    int32_t* id;
+    double* x; double* y;         /* position (top-left corner), centre = position + radius */
+    double* radius;
+    double* units;                /* resc_units   */
+    double* left;                 /* resc_left    */
+    double* quality;              /* unit_per_timestep */
+} abm_patch_state;
+
+typedef struct abm_forage_world abm_forage_world;   /* opaque */
+
+int abm_forage_world_create(abm_forage_batch* batch, const abm_forage_world_params* params_host,
+                            abm_forage_world** out);
+int abm_forage_world_destroy(abm_forage_world* world);
+/* patch_offset[n_replicas + 1] (host): each replica's first patch slot. */
+int abm_forage_world_get_offsets(const abm_forage_world* world, int64_t* patch_offset);
+
+/* Phases of abm_forage_world_run. */
+#define ABM_WORLD_STEP   0   /* interaction, regeneration, then Agent.update      */
+#define ABM_WORLD_RESUME 1   /* finish what a phase left waiting for words        */
+#define ABM_WORLD_FILL   2   /* add patches up to n_patches (create_resources,
+                                sims.py:538-542, ids max + 1)                     */
+/* Per-replica status after a run. */
+#define ABM_WORLD_DONE       0
+#define ABM_WORLD_NEED_WORDS 1   /* the word buffer ran out: nothing drawn, resume  */
+#define ABM_WORLD_TIMEOUT    2   /* more than 10000 placement retries (the reference
+                                    raises "Reached timeout while trying to create
+                                    resources without overlap!")                   */
+#define ABM_WORLD_BAD_COUNT  4   /* count[b] outside [0, n_patches]: nothing done    */
+
+/* One phase for every replica.  words: per replica the next raw 32-bit outputs
+ * of its numpy legacy MT19937 stream (abm_mt19937_peek32), word_offset[b] /
+ * word_count[b] locating them; patch regeneration draws from them first
+ * (randint by masked rejection, uniform from two words, as numpy's legacy
+ * RandomState does), then the random walk of Agent.update.  A STEP keeps
+ * 2 * n_agents words for the walk.  words_used[b] (device): words consumed;
+ * status[b] (device): ABM_WORLD_*.  A replica left NEED_WORDS has consumed
+ * nothing of this buffer; a RESUME with more words finishes it (and its
+ * Agent.update when the phase was a STEP) and leaves every other replica
+ * untouched.  pool_success: device int32 per agent (0 until notified).
+ * field_bits as in abm_forage_step_f64. */
+int abm_forage_world_run(abm_forage_world* world, int32_t phase, const abm_forage_state* state,
+                         int32_t* pool_success, const abm_patch_state* patches,
+                         const uint32_t* words, const int64_t* word_offset,
+                         const int32_t* word_count, int32_t* words_used, int32_t* status,
+                         uint32_t* field_bits, void* stream);
+
 /* ------------------------------------------------ host random numbers */
 /* numpy's legacy MT19937 state (numpy/random/src/mt19937/mt19937.h,
  * mt19937_state): the address a RandomState's bit generator reports as
@@ -360,6 +438,11 @@ typedef struct abm_mt19937_state {
 int abm_mt19937_peek(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts,
                      double* out);
 int abm_mt19937_skip(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts);
+/* The same for raw 32-bit outputs (next_uint32), the unit the resource loop's
+ * randint and uniform draws are made of (sims.py:349-359). */
+int abm_mt19937_peek32(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts,
+                       uint32_t* out);
+int abm_mt19937_skip32(abm_mt19937_state* const* states, int32_t n_states, const int64_t* counts);
 
 /* --------------------------------------------------------------- misc */
 const char* abm_last_error(void);

@@ -18,6 +18,11 @@ abm/agent/agent.py) and the host-loop pieces around it:
 * ``get_mode`` / ``set_mode`` agent.py:659-693
 * ``notify_agent``            abm/simulation/sims.py:29-42
 * ``deplete``                 abm/environment/rescource.py:118-133
+* ``ForageWorld``             the loop body of Simulation.start with agent
+                              collisions off (sims.py:789-861), patch creation
+                              and regeneration (sims.py:321-368, 538-542)
+* ``randint_from_words`` / ``uniform_from_words``  numpy's legacy bounded
+                              integers and uniforms from raw MT19937 outputs
 
 The random walk draws ``np.random.uniform`` from numpy's global legacy
 MT19937 stream in agent order, exactly as the reference does.
@@ -367,6 +372,33 @@ def deplete(resc, units):
         dep = resc["left"]
         resc["left"] = 0
     return dep, not (resc["left"] > 0)
+
+
+# ------------------------------------ numpy's legacy draws from raw words
+def randint_from_words(words, i, lo, hi):
+    """RandomState.randint(lo, hi) (numpy/random/_bounded_integers.pyx,
+    random_bounded_uint64_fill with use_masked): draw next_uint32 & mask until
+    the value is <= hi - 1 - lo, mask the smallest all-ones word covering it;
+    no draw for a single value.  words[i:] are the generator's next raw
+    outputs; returns (value, next i)."""
+    rng = hi - 1 - lo
+    if rng == 0:
+        return lo, i
+    mask = rng
+    for s in (1, 2, 4, 8, 16):
+        mask |= mask >> s
+    while True:
+        v = int(words[i]) & mask
+        i += 1
+        if v <= rng:
+            return lo + v, i
+
+
+def uniform_from_words(words, i, lo, hi):
+    """RandomState.uniform(lo, hi) = lo + (hi - lo) * random_sample, the
+    double made of two outputs (a >> 5) * 2^26 + (b >> 6), scaled by 2^-53."""
+    a, b = int(words[i]) >> 5, int(words[i + 1]) >> 6
+    return lo + (hi - lo) * ((a * 67108864.0 + b) / 9007199254740992.0), i + 2
 
 
 # ------------------------------------------------------- array helpers

@@ -110,6 +110,42 @@ def test_mt19937_peek_and_skip_follow_numpys_legacy_stream():
         assert g.uniform(-0.3, 0.3) == t.uniform(-0.3, 0.3)
 
 
+def test_mt19937_peek32_and_skip32_give_numpys_raw_words():
+    # the resource loop's words: peek32 must equal next_uint32 of a copy of
+    # each RandomState (randint over the full 32-bit range returns them
+    # unmasked), skip32 must leave it where that many words leave it
+    lib = L.load()
+    gens = [np.random.RandomState(100 + s) for s in range(12)]
+    for i, g in enumerate(gens):
+        g.random_sample((i * 71) % 400)
+    counts = np.array([(i * 97) % 1400 for i in range(12)], dtype=np.int64)
+    addr = _addrs(gens)
+    out = np.zeros(int(counts.sum()), dtype=np.uint32)
+    before = [g.get_state() for g in gens]
+    assert lib.abm_mt19937_peek32(addr.ctypes.data_as(C.c_void_p), len(gens),
+                                  counts.ctypes.data_as(C.c_void_p),
+                                  out.ctypes.data_as(C.c_void_p)) == 0
+    want = []
+    for g, c in zip(gens, counts):
+        t = np.random.RandomState()
+        t.set_state(g.get_state())
+        want.append(t.randint(0, 2 ** 32, size=int(c), dtype=np.uint32))
+    assert np.array_equal(out, np.concatenate(want))
+    assert lib.abm_mt19937_skip32(addr.ctypes.data_as(C.c_void_p), len(gens),
+                                  counts.ctypes.data_as(C.c_void_p)) == 0
+    for g, b, c in zip(gens, before, counts):
+        t = np.random.RandomState()
+        t.set_state(b)
+        t.randint(0, 2 ** 32, size=int(c), dtype=np.uint32)
+        assert g.get_state()[2] == t.get_state()[2]
+        assert np.array_equal(g.get_state()[1], t.get_state()[1])
+
+
+def test_world_struct_layouts():
+    assert C.sizeof(L.ForageWorldParams) == 8 * 4 + 3 * 8
+    assert C.sizeof(L.PatchState) == 8 * 8
+
+
 def test_mt19937_rejects_bad_arguments():
     lib = L.load()
     g = np.random.RandomState(1)

@@ -1,0 +1,209 @@
+"""GPU parity of the device resource loop (abm_forage_world_run through
+abm_amd.forage.ResourceWorld) against the oracle's ForageWorld, the loop body
+of Simulation.start with agent collisions off (sims.py:789-861) and patch
+creation / regeneration (sims.py:321-368, 538-542), each replica on its own
+numpy RandomState.
+
+Bar: patches (ids, positions, units, resources left, quality), modes,
+environment status, patch ids, novelty rings, pool_success, collected
+resources, fields and every generator's stream position identical after each
+step; continuous state (x, y, theta, v, w, u, I_priv, collected_r_before)
+within 1e-12 relative.  The state is reloaded from the oracle after each step
+so that every step starts from identical values (the bias towards the patch
+centre uses atan2, whose device and host results may differ in the last
+ulp)."""
+import numpy as np
+import pytest
+
+from oracle import forage_oracle as fo
+
+pytestmark = pytest.mark.gpu
+PI = np.pi
+
+DEC = dict(Eps_w=3.0, Eps_u=1.0, S_wu=0.25, S_uw=0.01, F_N=0.5, F_R=0.5, exp_vel_max=3.0,
+           exp_theta_min=-0.5, exp_theta_max=0.5, reloc_theta_max=1.8, exp_stop_ratio=0.175)
+
+# (N agents, arena, R, WorldConfig kwargs, mixed start)
+WORLDS = [
+    (30, 300, 256, dict(N_resc=4, patch_radius=30, min_resc_perpatch=3, max_resc_perpatch=8,
+                        min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=True), False),
+    (20, 300, 128, dict(N_resc=5, patch_radius=25, min_resc_perpatch=2, max_resc_perpatch=4,
+                        min_resc_quality=0.2, max_resc_quality=0.6, teleport_exploit=False,
+                        regenerate_patches=False, agent_consumption=0.5), True),
+    (25, 250, 256, dict(N_resc=8, patch_radius=30, min_resc_perpatch=4, max_resc_perpatch=6,
+                        min_resc_quality=0.7, max_resc_quality=-1.0, teleport_exploit=True,
+                        allow_border_patch_overlap=True), True),
+    (1, 200, 64, dict(N_resc=1, patch_radius=40, min_resc_perpatch=1, max_resc_perpatch=2,
+                      min_resc_quality=0.4, max_resc_quality=0.9, teleport_exploit=True), False),
+    # a positive pooling time: only relocating agents that were notified
+    # before (pool_success) hear of patches
+    (15, 250, 128, dict(N_resc=3, patch_radius=35, min_resc_perpatch=2, max_resc_perpatch=5,
+                        pooling_time=2, teleport_exploit=False), True),
+]
+
+
+def rel(a, b, s=1.0):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), s)))
+
+
+def make_agents(seed, N, W, R, mixed, pooling_time):
+    p = fo.ForageParams(**DEC)
+    rng = np.random.RandomState(seed)
+    ags = []
+    for i in range(N):
+        x = rng.randint(30 - 10, W + 30 - 10)
+        y = rng.randint(30 - 10, W + 30 - 10)
+        ags.append(fo.ForageAgent(i, 10, (x, y), rng.uniform(0, 2 * PI), R, (-PI, PI), W, W, 30,
+                                  150.0, False, True, p, pooling_time=pooling_time))
+    if mixed:
+        for a in ags:
+            a.w = float(rng.uniform(-0.2, 1.0))
+            a.u = float(rng.uniform(-0.2, 1.0))
+            a.velocity = float(rng.uniform(0, 3))
+            a.set_mode(str(rng.choice(["explore", "exploit", "relocate"], p=[0.5, 0.3, 0.2])))
+            a.pool_success = int(rng.rand() < 0.5)
+    return ags, p
+
+
+def forage_config(N, W, R, p):
+    from abm_amd.forage import ForageConfig
+    return ForageConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=30, fov=(-PI, PI),
+                        vision_range=150.0, visual_exclusion=False, patchwise_exclusion=True,
+                        T_w=p.T_w, Eps_w=p.Eps_w, g_w=p.g_w, B_w=p.B_w, w_max=p.w_max, T_u=p.T_u,
+                        Eps_u=p.Eps_u, g_u=p.g_u, B_u=p.B_u, u_max=p.u_max, S_wu=p.S_wu,
+                        S_uw=p.S_uw, Tau=p.Tau, F_N=p.F_N, F_R=p.F_R, exp_vel_max=p.exp_vel_max,
+                        exp_theta_min=p.exp_theta_min, exp_theta_max=p.exp_theta_max,
+                        reloc_theta_max=p.reloc_theta_max, exp_stop_ratio=p.exp_stop_ratio)
+
+
+def oracle_world(ags, W, wk):
+    from abm_amd.forage import WorldConfig
+    c = WorldConfig(**wk)
+    return fo.ForageWorld(ags, W, W, 30, c.N_resc, c.patch_radius, c.min_resc_perpatch,
+                          c.max_resc_perpatch, c.min_resc_quality, c.max_resc_quality,
+                          regenerate=c.regenerate_patches, teleport_exploit=c.teleport_exploit,
+                          allow_border_overlap=c.allow_border_patch_overlap,
+                          agent_consumption=c.agent_consumption)
+
+
+def load_agents(sw, rw, ags, b):
+    sl = sw.replica_slice(b)
+    arr = fo.agents_arrays(ags)
+    st = sw.get_state()
+    for k in ("x", "y", "theta", "v", "w", "u", "I_priv", "collected_r", "collected_r_before",
+              "mode", "overriding_mode", "env_status", "exploited_patch_id"):
+        st[k][sl] = arr[k]
+    nov = st["novelty"].reshape(-1, sw.tau)
+    nov[sl] = arr["novelty"]
+    st["novelty"] = nov.reshape(-1)
+    sw.set_state(**{k: v for k, v in st.items() if k != "radius"})
+    ps = rw.pool_success.cpu().numpy()
+    ps[sl] = [a.pool_success for a in ags]
+    rw.pool_success.copy_(rw.pool_success.new_tensor(ps))
+
+
+def compare(sw, rw, world, b, where):
+    sl = sw.replica_slice(b)
+    st = sw.get_state()
+    arr = fo.agents_arrays(world.agents)
+    for k in ("mode", "overriding_mode", "env_status", "exploited_patch_id", "collected_r"):
+        assert np.array_equal(st[k][sl], arr[k]), (where, k)
+    assert np.array_equal(st["novelty"].reshape(-1, sw.tau)[sl], arr["novelty"]), where
+    assert np.array_equal(rw.pool_success.cpu().numpy()[sl], [a.pool_success for a in world.agents])
+    for k in ("x", "y", "theta", "v", "w", "u", "I_priv", "collected_r_before"):
+        assert rel(st[k][sl], arr[k]) <= 1e-12, (where, k)
+    got = rw.patches(b)
+    exp = dict(id=[p.id for p in world.patches], x=[p.position[0] for p in world.patches],
+               y=[p.position[1] for p in world.patches], radius=[p.radius for p in world.patches],
+               units=[p.resc_units for p in world.patches], left=[p.resc_left for p in world.patches],
+               quality=[p.unit_per_timestep for p in world.patches])
+    for k, v in exp.items():
+        assert np.array_equal(got[k], np.asarray(v, dtype=got[k].dtype)), (where, k, got[k], v)
+
+
+def same_stream(a, b):
+    return a.get_state()[2] == b.get_state()[2] and np.array_equal(a.get_state()[1], b.get_state()[1])
+
+
+@pytest.mark.parametrize("regen_words", [32, 1])
+def test_world_matches_oracle(gpu, regen_words):
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    worlds, cfgs, wcfgs, allags = [], [], [], []
+    for b, (N, W, R, wk, mixed) in enumerate(WORLDS):
+        ags, p = make_agents(70 + b, N, W, R, mixed, wk.get("pooling_time", 0))
+        worlds.append((ags, W, wk))
+        cfgs.append(forage_config(N, W, R, p))
+        wcfgs.append(WorldConfig(**wk))
+        allags += ags
+    rngs = [np.random.RandomState(900 + b) for b in range(len(WORLDS))]
+    orngs = [np.random.RandomState(900 + b) for b in range(len(WORLDS))]
+    sw = ForageSwarm(cfgs, [a.position[0] for a in allags], [a.position[1] for a in allags],
+                     [a.orientation for a in allags], radius=10, rngs=rngs)
+    rw = ResourceWorld(sw, wcfgs, regen_words=regen_words)
+    oracle = []
+    for b, (ags, W, wk) in enumerate(worlds):
+        load_agents(sw, rw, ags, b)
+        np.random.set_state(orngs[b].get_state())
+        oracle.append(oracle_world(ags, W, wk))
+        orngs[b].set_state(np.random.get_state())
+    print(f"create_resources regen_words={regen_words}", flush=True)
+    rw.create_resources()
+    for b, world in enumerate(oracle):
+        assert same_stream(rngs[b], orngs[b]), b
+        compare(sw, rw, world, b, ("create", b))
+    kills = [0] * len(oracle)
+    first = [[(p.id, p.position[0], p.position[1]) for p in w.patches] for w in oracle]
+    for step in range(80):
+        rw.step(fields=True)
+        for b, world in enumerate(oracle):
+            before = [(p.id, p.position[0], p.position[1]) for p in world.patches]
+            np.random.set_state(orngs[b].get_state())
+            world.step()
+            orngs[b].set_state(np.random.get_state())
+            after = [(p.id, p.position[0], p.position[1]) for p in world.patches]
+            kills[b] += len(set(before) - set(after))
+            assert same_stream(rngs[b], orngs[b]), (step, b)
+            compare(sw, rw, world, b, (step, b))
+            assert np.array_equal(sw.fields_numpy(b),
+                                  (fo.agents_arrays(world.agents)["soc_v_field"] != 0).astype(np.uint8))
+            load_agents(sw, rw, world.agents, b)
+    # depletion, regeneration (ids kept) and removal all happened
+    assert kills[0] > 0 and kills[2] > 0 and kills[3] > 0, kills
+    assert len(oracle[1].patches) < len(first[1])
+    assert sorted(p.id for p in oracle[0].patches) == sorted(i for i, _, _ in first[0])
+    if regen_words == 1:
+        assert rw.resumes > 0
+
+
+def test_world_placement_timeout_raises(gpu):
+    # two patches of radius 30 whose centres must lie in a 40 x 40 square
+    # always collide: the reference gives up after 10000 retries
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig, TIMEOUT_MESSAGE
+    ags, p = make_agents(5, 1, 100, 16, False, 0)
+    wk = dict(N_resc=2, patch_radius=30, min_resc_perpatch=5, max_resc_perpatch=10)
+    np.random.seed(11)
+    with pytest.raises(Exception, match=TIMEOUT_MESSAGE):
+        oracle_world(ags, 100, wk)
+    np.random.seed(11)
+    sw = ForageSwarm([forage_config(1, 100, 16, p)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=10)
+    rw = ResourceWorld(sw, [WorldConfig(**wk)])
+    with pytest.raises(Exception, match=TIMEOUT_MESSAGE):
+        rw.create_resources()
+
+
+def test_world_rejects_bad_patch_count(gpu):
+    from abm_amd import _lib as L
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    ags, p = make_agents(6, 3, 200, 16, False, 0)
+    sw = ForageSwarm([forage_config(3, 200, 16, p)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=10,
+                     rngs=[np.random.RandomState(1)])
+    rw = ResourceWorld(sw, [WorldConfig(N_resc=2)])
+    rw.p["count"][0] = 3
+    with pytest.raises(L.AbmError, match="patch count"):
+        rw.step()
+    with pytest.raises(L.AbmError):
+        ResourceWorld(sw, [WorldConfig(N_resc=65)])
+    with pytest.raises(L.AbmError):
+        ResourceWorld(sw, [WorldConfig(N_resc=2, patch_radius=120)])   # randint(30, -10)

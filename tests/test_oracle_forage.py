@@ -206,3 +206,110 @@ def test_notify_and_deplete():
     r = {"quality": 0.6, "left": 1.0}
     assert fo.deplete(r, 1) == (0.6, False)
     assert fo.deplete(r, 1) == (0.4, True)
+
+
+def _raw_words(g, n):
+    t = np.random.RandomState()
+    t.set_state(g.get_state())
+    return t.randint(0, 2 ** 32, size=n, dtype=np.uint32)
+
+
+def test_word_draws_follow_numpys_randint_and_uniform():
+    # the placement draws (sims.py:349-359) restated over raw MT19937 outputs
+    # must give numpy's own randint / uniform values and consume as many words
+    bounds = [(30, 440), (0, 1), (5, 6), (-30, 470), (200, 1000), (0, 2 ** 31), (7, 2 ** 32 - 1)]
+    for seed in range(20):
+        g = np.random.RandomState(seed)
+        g.random_sample(seed * 13)
+        words = _raw_words(g, 4000)
+        i = 0
+        for k in range(60):
+            lo, hi = bounds[(seed + k) % len(bounds)]
+            v, i = fo.randint_from_words(words, i, lo, hi)
+            assert v == g.randint(lo, hi), (seed, k, lo, hi)
+            q, i = fo.uniform_from_words(words, i, 0.1, 0.9)
+            assert q == g.uniform(0.1, 0.9)
+        assert np.array_equal(_raw_words(g, 3), words[i:i + 3])   # same stream position
+
+
+def _world(agents, W=300, n_resc=3, radius=30, lo=3, hi=6, **kw):
+    return fo.ForageWorld(agents, W, W, 30, n_resc, radius, lo, hi, 0.5, 1.0, **kw)
+
+
+def _agents(seed, N=12, W=300):
+    p = fo.ForageParams(Eps_u=1.0, F_N=0.5, F_R=0.5)
+    rng = np.random.RandomState(seed)
+    return [fo.ForageAgent(i, 10, (rng.randint(20, W + 20), rng.randint(20, W + 20)),
+                           rng.uniform(0, 2 * np.pi), 64, (-np.pi, np.pi), W, W, 30, 150.0, False,
+                           True, p) for i in range(N)]
+
+
+def test_world_patches_never_overlap_and_keep_ids():
+    # placement rule (proove_sprite with collide_circle): integer centres more
+    # than r1 + r2 apart, at creation and after every regeneration; a
+    # regenerated patch keeps the killed one's id
+    np.random.seed(4)
+    ags = _agents(4, N=30)
+    w = _world(ags, n_resc=5)
+    ids0 = sorted(p.id for p in w.patches)
+    assert ids0 == [1, 2, 3, 4, 5]
+    for a in ags:                                   # everyone exploits
+        a.u, a.env_status = 1.0, 1
+        a.set_mode("exploit")
+    moved = 0
+    for step in range(60):
+        pos = {p.id: tuple(p.position) for p in w.patches}
+        w.step()
+        moved += sum(pos[p.id] != tuple(p.position) for p in w.patches)
+        assert sorted(p.id for p in w.patches) == ids0
+        for i, p in enumerate(w.patches):
+            for q in w.patches[i + 1:]:
+                d2 = (p.center[0] - q.center[0]) ** 2 + (p.center[1] - q.center[1]) ** 2
+                assert d2 > (p.radius + q.radius) ** 2
+    assert moved > 0
+
+
+def test_world_conserves_resources():
+    # what the patches lose each step is what the agents collect
+    np.random.seed(8)
+    ags = _agents(8, N=25)
+    w = _world(ags, n_resc=4, lo=20, hi=40)
+    for a in ags:
+        a.u, a.env_status = 1.0, 1
+        a.set_mode("exploit")
+    for step in range(40):
+        left = {p.id: p.resc_left for p in w.patches}
+        got = sum(a.collected_r for a in ags)
+        w.step()
+        new = {p.id: p.resc_left for p in w.patches}
+        # a killed-and-regenerated patch gave all it had left
+        lost = sum(left[i] - (new[i] if new[i] <= left[i] else 0.0) for i in left)
+        assert abs(sum(a.collected_r for a in ags) - got - lost) <= 1e-9
+
+
+def test_world_single_exploiter_known_answers():
+    # one agent on one patch: 2.5 units, quality 1, consumption 1 -> it
+    # collects 1, 1, 0.5; the third depletion kills the patch, the agent is
+    # told -1 (twice rolled in: once as the depleting agent, once by the
+    # notify-all), and the patch is regenerated with the same id
+    np.random.seed(2)
+    ag = _agents(2, N=1)[0]
+    w = _world([ag], n_resc=1)
+    p = w.patches[0]
+    p.resc_units = p.resc_left = 2.5
+    p.unit_per_timestep = 1.0
+    ag.position = np.array([p.center[0] - 10.0, p.center[1] - 10.0])   # centre on the patch centre
+    ag.u, ag.env_status, ag.velocity = 1.0, 1, 0.0
+    ag.set_mode("exploit")
+    ag.novelty = np.zeros(ag.novelty.shape)
+    collected = []
+    for step in range(3):
+        before = ag.collected_r
+        ag.position = np.array([w.patches[0].center[0] - 10.0, w.patches[0].center[1] - 10.0])
+        ag.set_mode("exploit")
+        w.step()
+        collected.append(ag.collected_r - before)
+    assert collected == [1.0, 1.0, 0.5]
+    assert w.patches[0].id == 1 and w.patches[0].resc_left == w.patches[0].resc_units
+    assert ag.env_status == -1 and ag.exploited_patch_id == -1
+    assert ag.novelty[0] == 0 and ag.novelty[1] == 0
