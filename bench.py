@@ -170,7 +170,7 @@ def other_workload(args):
     """Measurements of the other configurations (not the headline line)."""
     import torch
     from abm_amd.vf import VFConfig, VFSwarm
-    from abm_amd.forage import ForageConfig, ForageSwarm
+    from abm_amd.forage import ForageConfig, ForageSwarm, ResourceWorld, WorldConfig
     f32 = lambda a: np.asarray(a).astype(np.float32).astype(np.float64)
     if args.workload == "single":
         Ns, Rs, Ws = 100, 1024, 1000.0
@@ -237,23 +237,30 @@ def other_workload(args):
         vsw = VFSwarm(vf_cfgs, cat(vx, 0), cat(vx, 1), cat(vx, 2), 0.5, radius=5, precision="f32")
         fsw = ForageSwarm(fg_cfgs, cat(fx_, 0), cat(fx_, 1), cat(fx_, 2), radius=10,
                           rngs=[np.random.RandomState(30000 + i) for i in range(len(fg_cfgs))])
+        # each foraging replica runs the whole loop body: 10 patches of radius
+        # 30, quality U[0.5, 1], 200..1000 units, regenerated when depleted
+        world = ResourceWorld(fsw, [WorldConfig(N_resc=10, patch_radius=30, min_resc_quality=0.5,
+                                                max_resc_quality=1.0, teleport_exploit=False)
+                                    for _ in fg_cfgs])
+        world.create_resources()
         vsw.autotune(trial_steps=2)
         ms_vf = time_steps(vsw, args.steps, args.warmup, not args.no_graph)[0]
-        # the foraging step reads back the uniforms it used each step: eager launches
+        # a foraging step reads back the words it used: eager launches
         for _ in range(min(args.warmup, 3)):
-            fsw.step()
+            world.step()
         torch.cuda.synchronize()
         n_fs = max(3, min(args.steps, 20))
         t0 = time.perf_counter()
         for _ in range(n_fs):
-            fsw.step()
+            world.step()
         torch.cuda.synchronize()
         ms_fg = (time.perf_counter() - t0) * 1e3 / n_fs
         n_vf = int(vsw.batch.info.n_agents_total)
         n_fg = int(fsw.info.n_agents_total)
         ms = ms_vf + ms_fg
         _line(args, "2000 ragged replicas (N log-uniform 8..512, R in {256,512,1024}, side 100*sqrt(N)): "
-                    "odd replicas flock (fused float32), even replicas forage (float64)",
+                    "odd replicas flock (fused float32), even replicas forage (float64) among 10 "
+                    "resource patches each (interaction, regeneration and update on the device)",
               (n_vf + n_fg) / (ms * 1e-3), ms, "f32+f64",
               {"replicas": 2000, "agents_flocking": n_vf, "agents_foraging": n_fg},
               flocking={"agent_steps_per_s": n_vf / (ms_vf * 1e-3), "ms_per_step": ms_vf},
