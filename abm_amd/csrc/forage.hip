@@ -117,6 +117,7 @@ __device__ inline void paint_record(uint8_t* V, const Rec& rc, int R, uint8_t wr
 // Per-replica constants in device memory.
 struct FGRep {
     int32_t agent_offset, n_agents, n_rays, words, visual_exclusion, patchwise, tau, pad_;
+    int32_t keep_lo, keep_hi;   // soc_v_field indices the FOV mask keeps (a contiguous range)
     int64_t field_offset;
     double fov0, fov1, width, height, pad, vision_range, step;
     double T_w, Eps_w, g_w, B_w, w_max, T_u, Eps_u, g_u, B_u, u_max, S_wu, S_uw, F_N, F_R;
@@ -130,33 +131,43 @@ __device__ inline int get_mode(int overriding, double w, double T_w) {
 }
 
 struct FGSmem {
+    Focal* focal;   // step kernel: focal geometry of the next two agents, by parity
     double *x, *y, *th, *v, *w, *u, *cr, *crb, *nov, *rad;
     int *mode, *over, *env, *pid, *cls;
     Rec* rec;
     int* sorted;
+    int* lst;       // step kernel: obstacles needing records, two lists of maxN by parity
+    int* cls2;      // step kernel: classes for the agent after next (cls holds the next one's)
+    double* uni;    // step kernel: the step's uniforms (or pairs of words), staged
     uint8_t* V;
-    int* cnt;       // [0] total, [1] left half
+    int* cnt;       // step kernel: [2p] total, [2p+1] left half by parity; [4 + p] list sizes
     double* lastd;  // keep_distance: distance of the last processed obstacle
 };
 
 __device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
     FGSmem s;
     char* p = reinterpret_cast<char*>(base);
+    s.focal = reinterpret_cast<Focal*>(p); p += 128;
     double** dptrs[] = {&s.x, &s.y, &s.th, &s.v, &s.w, &s.u, &s.cr, &s.crb, &s.nov, &s.rad};
     for (double** d : dptrs) { *d = reinterpret_cast<double*>(p); p += sizeof(double) * maxN; }
     s.lastd = reinterpret_cast<double*>(p); p += sizeof(double) * 2;
+    s.uni = reinterpret_cast<double*>(p); p += sizeof(double) * maxN;
     s.rec = reinterpret_cast<Rec*>(p); p += sizeof(Rec) * maxN;
     int** iptrs[] = {&s.mode, &s.over, &s.env, &s.pid, &s.cls};
     for (int** q : iptrs) { *q = reinterpret_cast<int*>(p); p += sizeof(int) * maxN; }
     s.sorted = reinterpret_cast<int*>(p); p += sizeof(int) * maxN;
-    s.cnt = reinterpret_cast<int*>(p); p += sizeof(int) * 4;
+    s.lst = reinterpret_cast<int*>(p); p += sizeof(int) * 2 * maxN;
+    s.cls2 = reinterpret_cast<int*>(p); p += sizeof(int) * maxN;
+    s.cnt = reinterpret_cast<int*>(p); p += sizeof(int) * 8;
     s.V = reinterpret_cast<uint8_t*>(p);
     return s;
 }
 
+static_assert(sizeof(Focal) <= 64, "two Focal slots of 64 bytes");
+
 size_t fg_lds_bytes(int maxN, int maxR) {
-    return sizeof(double) * ((size_t)10 * maxN + 2) + sizeof(Rec) * (size_t)maxN +
-           sizeof(int) * ((size_t)6 * maxN + 4) + (size_t)maxR + 16;
+    return 128 + sizeof(double) * ((size_t)11 * maxN + 2) + sizeof(Rec) * (size_t)maxN +
+           sizeof(int) * ((size_t)9 * maxN + 8) + (size_t)maxR + 16;
 }
 
 // Records of one focal agent against the obstacle set marked in S.cls
@@ -263,6 +274,210 @@ __device__ inline int soc_bit(const uint8_t* V, int k, int R, double step, doubl
     return V[R - 1 - k];
 }
 
+// Cue class of agent j for focal agent f (agent.py:396-413): in range by
+// supcalc.distance; exploiting agents of another patch are the cues, those
+// of the focal's own patch occlude (patchwise exclusion), exploiting agents
+// without a patch are dropped; the rest in range (the focal included) occlude.
+__device__ inline int cue_class(const FGSmem& S, const FGRep& P, int j, int f, double cfx, double cfy,
+                                int pid_f) {
+#pragma clang fp contract(off)
+    const double dx = (S.x[j] + S.rad[j]) - cfx, dy = (S.y[j] + S.rad[j]) - cfy;
+    const bool in_range = norm2(dx, dy) <= P.vision_range;
+    const bool expl = in_range && j != f && get_mode(S.over[j], S.w[j], P.T_w) == ABM_MODE_EXPLOIT;
+    int c = kNone;
+    if (expl) {
+        if (P.patchwise && S.pid[j] == pid_f) c = kSamePatch;
+        else if (S.pid[j] != -1) c = kSocial;
+    } else if (in_range) {
+        c = kNonExpl;
+    }
+    return c;
+}
+
+// Whether the raster needs obstacle j's record: the cues, and with visual
+// exclusion every obstacle in range (agent.py:415-419).
+__device__ inline bool needs_record(int c, bool excl) { return c == kSocial || (excl && c != kNone); }
+
+// Record of obstacle j (class c) for focal geometry f.
+__device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Focal& f, int R, double step,
+                                double fov0, double fov1) {
+    Rec rc;
+    rc.valid = 0; rc.social = 0; rc.order = 0; rc.start = rc.end = rc.sx = rc.ex = 0;
+    rc.dist = 0.0; rc.vis = 0.0;
+    const PairFG p = pair_forage_exact(f, S.x[j], S.y[j], R, step, fov0, fov1);
+    if (!p.skipped && p.visible) {
+        rc.valid = 1;
+        rc.social = c == kSocial;
+        rc.order = c * N + j;
+        rc.start = rc.sx = p.start;
+        rc.end = rc.ex = p.end;
+        rc.dist = p.dist;
+        rc.vis = p.vis;
+    }
+    return rc;
+}
+
+// Classes of the agents j != skip for focal agent f into S.cls; those whose
+// records the raster needs are appended to lst (in any order: records are
+// kept per agent and ranked by (distance, list position)).  Thread t0 of
+// `stride` threads, both multiples of 64 apart per wave.
+__device__ void classify(const FGSmem& S, const FGRep& P, int N, int f, int skip1, int skip2, bool excl,
+                         int* cls, int* lst, int* lst_n, int t0, int stride) {
+    const double cfx = S.x[f] + S.rad[f], cfy = S.y[f] + S.rad[f];
+    const int pid_f = S.pid[f];
+    const int lane = t0 & 63;
+    for (int base = t0 - lane; base < N; base += stride) {
+        const int j = base + lane;
+        bool need = false;
+        if (j < N && j != skip1 && j != skip2) {
+            const int c = cue_class(S, P, j, f, cfx, cfy, pid_f);
+            cls[j] = c;
+            need = needs_record(c, excl);
+        }
+        const uint64_t m = __ballot(need);
+        if (m) {
+            int at = 0;
+            if (lane == 0) at = atomicAdd(lst_n, (int)__popcll(m));
+            at = __builtin_amdgcn_readfirstlane(at);
+            if (need) lst[at + (int)__popcll(m & ((1ull << lane) - 1ull))] = j;
+        }
+    }
+}
+
+// make_focal with one sincos for the heading (the same values as cos, sin).
+__device__ inline Focal focal_of(const FGSmem& S, int k) {
+#pragma clang fp contract(off)
+    const double px = S.x[k], py = S.y[k], radius = S.rad[k];
+    double sn, cs;
+    sincos(S.th[k], &sn, &cs);
+    Focal f;
+    f.px = px; f.py = py; f.radius = radius;
+    f.cx = px + radius;
+    f.cy = py + radius;
+    const double ex = px + (1.0 + cs) * radius;
+    const double ey = py + (1.0 - sn) * radius;
+    const double v1x = ex - f.cx, v1y = ey - f.cy;
+    const double n1 = norm2(v1x, v1y);
+    f.u1x = v1x / n1;
+    f.u1y = v1y / n1;
+    return f;
+}
+
+// Set bits [lo, hi) of a bit-packed field.
+__device__ inline void or_range(uint32_t* Vb, int lo, int hi) {
+    while (lo < hi) {
+        const int w = lo >> 5, b0 = lo & 31;
+        const int end = min(hi, (w + 1) * 32);
+        const int nb = end - lo;
+        atomicOr(&Vb[w], nb == 32 ? 0xffffffffu : ((1u << nb) - 1u) << b0);
+        lo = end;
+    }
+}
+
+// paint_record for the step kernel's bit-packed raw field: every social
+// record paints 1, so the pieces are ORed in any order (agent.py:577-590).
+__device__ inline void paint_bits(uint32_t* Vb, const Rec& rc, int R) {
+    int s = rc.sx, e = rc.ex, lo, hi;
+    if (s < 0) {
+        py_slice(R + s, R, R, &lo, &hi);
+        or_range(Vb, lo, hi);
+        s = 0;
+    }
+    if (e >= R) {
+        py_slice(0, e - R, R, &lo, &hi);
+        or_range(Vb, lo, hi);
+        e = R - 1;
+    }
+    py_slice(s, e, R, &lo, &hi);
+    or_range(Vb, lo, hi);
+}
+
+// Bits of word w inside the inclusive index range [a, b].
+__device__ inline uint32_t word_range_mask(int w, int a, int b) {
+    const int lo = max(a, w * 32), hi = min(b, w * 32 + 31);
+    if (lo > hi) return 0u;
+    const int nb = hi - lo + 1;
+    return nb == 32 ? 0xffffffffu : ((1u << nb) - 1u) << (lo & 31);
+}
+
+// Visual exclusion (agent.py:421-445) over the listed records, then their
+// raster (agent.py:569-590) into the bit-packed raw field S.V (uint32
+// words, bit r = V[r]).  Every thread; ends synchronised.
+__device__ void raster_list(const FGSmem& S, int R, bool excl, const int* lst, int n0, int extra) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    // the listed records, and one more (extra >= 0) kept outside the list
+    const int n = n0 + (extra >= 0 ? 1 : 0);
+    auto item = [&](int q) { return q < n0 ? lst[q] : extra; };
+    if (excl) {
+        // rank = position in the stable ascending sort by distance
+        for (int q = tid; q < n; q += nthr) {
+            const int j = item(q);
+            const Rec rf = S.rec[j];
+            if (!rf.valid) continue;
+            int rank = 0;
+            for (int p = 0; p < n; ++p) {
+                const Rec ri = S.rec[item(p)];
+                if (ri.valid && (ri.dist < rf.dist || (ri.dist == rf.dist && ri.order < rf.order))) ++rank;
+            }
+            S.sorted[rank] = j;
+        }
+        __syncthreads();
+        for (int q = tid; q < n; q += nthr) {
+            const int j = item(q);
+            Rec rf = S.rec[j];
+            if (!rf.valid) continue;
+            // walk the nearer records in rank order; rank 0 keeps its interval
+            for (int p = 0; p < n; ++p) {
+                const int i = S.sorted[p];
+                if (i == j) break;
+                const Rec ri = S.rec[i];
+                if (ri.dist < rf.dist) {
+                    if (rf.sx <= ri.start && ri.start <= rf.ex) rf.ex = ri.start;
+                    if (rf.sx <= ri.end && ri.end <= rf.ex) rf.sx = ri.end;
+                    if (ri.start <= rf.sx && ri.end >= rf.ex) { rf.sx = 0; rf.ex = 0; }
+                }
+            }
+            S.rec[j].sx = rf.sx;
+            S.rec[j].ex = rf.ex;
+        }
+        __syncthreads();
+    }
+    uint32_t* Vb = reinterpret_cast<uint32_t*>(S.V);
+    for (int q = tid; q < n; q += nthr) {
+        const Rec rc = S.rec[item(q)];
+        if (rc.valid && rc.social) paint_bits(Vb, rc, R);
+    }
+    __syncthreads();
+}
+
+// The returned field of agent k: counts for mean() and the left/right
+// halves into cnt; soc_v_field[r] = V[R-1-r] on the rays the FOV mask keeps.
+__device__ void count_field(const FGSmem& S, const FGRep& P, int R, int* cnt, uint32_t* bits_out, int k) {
+    const int tid = threadIdx.x, nthr = blockDim.x, nl = R / 2, W = P.words;
+    const uint32_t* Vb = reinterpret_cast<const uint32_t*>(S.V);
+    // soc_v_field[q] = V[R-1-q] for keep_lo <= q <= keep_hi: raw indices
+    // [R-1-keep_hi, R-1-keep_lo]; the left half q < R/2 is raw r >= R - R/2
+    const int ta = R - 1 - P.keep_hi, tb = R - 1 - P.keep_lo, la = max(R - nl, ta);
+    for (int w = tid; w < W; w += nthr) {
+        const uint32_t bits = Vb[w];
+        const int tot = __popc(bits & word_range_mask(w, ta, tb));
+        const int left = __popc(bits & word_range_mask(w, la, tb));
+        if (tot) atomicAdd(&cnt[0], tot);
+        if (left) atomicAdd(&cnt[1], left);
+    }
+    if (bits_out) {
+        uint32_t* dst = bits_out + P.field_offset + (int64_t)k * W;
+        for (int wd = tid; wd < W; wd += nthr) {
+            uint32_t m = 0;
+            for (int b = 0; b < 32; ++b) {
+                const int q = wd * 32 + b, r = R - 1 - q;
+                if (q >= P.keep_lo && q <= P.keep_hi && ((Vb[r >> 5] >> (r & 31)) & 1u)) m |= 1u << b;
+            }
+            dst[wd] = m;
+        }
+    }
+}
+
 // Status of a replica between the interaction kernel and the update.
 constexpr int kWorldReady = 3;
 
@@ -301,64 +516,66 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         S.mode[i] = st.mode[g]; S.over[i] = st.overriding_mode[g];
         S.env[i] = st.env_status[g]; S.pid[i] = st.exploited_patch_id[g];
     }
-    const double* ubuf = words ? nullptr : uniforms + u_off[blockIdx.x];
-    const uint32_t* wbuf = words ? words + u_off[blockIdx.x] : nullptr;
+    // the step's uniforms, staged in LDS: agent i's draw is the i-th (at most
+    // one per agent); doubles, or pairs of words after those the resource
+    // loop used (u_used[b] on entry); NaN past the end of the buffer
     const int ucount = u_cnt[blockIdx.x];
-    int used = words ? u_used[blockIdx.x] : 0;   // meaningful in thread 0
+    const int used0 = words ? u_used[blockIdx.x] : 0;
+    for (int i = tid; i < N; i += nthr) {
+        double uu = NAN;
+        if (words) {
+            const uint32_t* wb = words + u_off[blockIdx.x];
+            const int at = used0 + 2 * i;
+            if (at + 2 <= ucount) uu = words_to_double(wb[at], wb[at + 1]);
+        } else if (i < ucount) {
+            uu = uniforms[u_off[blockIdx.x] + i];
+        }
+        S.uni[i] = uu;
+    }
+    int draws = 0;   // meaningful in thread 0
     const int nl = R / 2;
+    const bool excl = P.visual_exclusion != 0;
+    int* const lst[2] = {S.lst, S.lst + maxN};
+    int* const clsb[2] = {S.cls, S.cls2};
+    int* const lst_n = S.cnt + 4;
+    // Pipeline over the agents in id order.  Agent m's field depends on the
+    // agents moved before it only through their records.  So while thread 0
+    // moves agent k and then adds agent k's class (and record) for agent
+    // k+1, wave 1 builds agent k+1's records of every other obstacle and
+    // waves 2 and 3 classify agent k+2's obstacles but k and k+1.  After one
+    // barrier come exclusion, raster and counts of agent k+1, during which
+    // wave 2 adds agent k's class for agent k+2 and wave 3 builds agent
+    // k+2's focal geometry (agents move only in their own turn, so it is the
+    // start-of-step pose).  Classes, lists, focal geometry and counts
+    // alternate between two buffers by parity of the focal agent.
+    const int rbase = 64, cbase = 128, cstride = nthr - 128;   // blockDim 256
+    if (tid == 0) {
+        for (int q = 0; q < 8; ++q) S.cnt[q] = 0;
+    }
+    for (int w = tid; w < P.words; w += nthr) reinterpret_cast<uint32_t*>(S.V)[w] = 0u;
+    __syncthreads();
+    // ---- agent 0's field; agent 1's classes but agent 0's
+    if (tid == 0) S.focal[0] = focal_of(S, 0);
+    if (tid == cbase && N > 1) S.focal[1] = focal_of(S, 1);
+    classify(S, P, N, 0, -1, -1, excl, clsb[0], lst[0], &lst_n[0], tid, nthr);
+    if (N > 1) classify(S, P, N, 1, 0, -1, excl, clsb[1], lst[1], &lst_n[1], tid, nthr);
+    __syncthreads();
+    for (int q = tid; q < lst_n[0]; q += nthr) {
+        const int j = lst[0][q];
+        S.rec[j] = record_of(S, j, clsb[0][j], N, S.focal[0], R, P.step, P.fov0, P.fov1);
+    }
+    __syncthreads();
+    raster_list(S, R, excl, lst[0], lst_n[0], -1);
+    if (tid == 0) lst_n[0] = 0;   // rebuilt for agent 2
+    count_field(S, P, R, S.cnt, bits_out, 0);
     __syncthreads();
 
     for (int k = 0; k < N; ++k) {
-        // ---- cue classes (agent.py:396-413)
-        const double ckx = S.x[k] + S.rad[k], cky = S.y[k] + S.rad[k];
-        const int pid_k = S.pid[k];
-        for (int j = tid; j < N; j += nthr) {
-            const double dx = (S.x[j] + S.rad[j]) - ckx, dy = (S.y[j] + S.rad[j]) - cky;
-            const bool in_range = norm2(dx, dy) <= P.vision_range;
-            const bool expl = in_range && j != k &&
-                              get_mode(S.over[j], S.w[j], P.T_w) == ABM_MODE_EXPLOIT;
-            int c = kNone;
-            if (expl) {
-                if (P.patchwise && S.pid[j] == pid_k) c = kSamePatch;
-                else if (S.pid[j] != -1) c = kSocial;
-            } else if (in_range) {
-                c = kNonExpl;
-            }
-            S.cls[j] = c;
-        }
-        __syncthreads();
-        const Focal f = make_focal(S.x[k], S.y[k], S.th[k], S.rad[k]);
-        const bool excl = P.visual_exclusion != 0;
-        project_focal(S, N, R, P.step, P.fov0, P.fov1, excl, excl, f);
-
-        // ---- the returned field: counts for mean() and the left/right halves
-        if (tid == 0) { S.cnt[0] = 0; S.cnt[1] = 0; }
-        __syncthreads();
-        int tot = 0, left = 0;
-        for (int r = tid; r < R; r += nthr) {
-            const int b = soc_bit(S.V, r, R, P.step, P.fov0, P.fov1);
-            tot += b;
-            if (r < nl) left += b;
-        }
-        if (tot) atomicAdd(&S.cnt[0], tot);
-        if (left) atomicAdd(&S.cnt[1], left);
-        if (bits_out) {
-            const int W = P.words;
-            uint32_t* dst = bits_out + P.field_offset + (int64_t)k * W;
-            for (int wd = tid; wd < W; wd += nthr) {
-                uint32_t m = 0;
-                for (int b = 0; b < 32; ++b) {
-                    const int r = wd * 32 + b;
-                    if (r < R && soc_bit(S.V, r, R, P.step, P.fov0, P.fov1)) m |= 1u << b;
-                }
-                dst[wd] = m;
-            }
-        }
-        __syncthreads();
-
+        const int f1 = k + 1, par = f1 & 1, f2 = k + 2, par2 = par ^ 1;
+        const int* cnt = S.cnt + 2 * (k & 1);
         // ---- decision and motion (agent.py:168-283)
         if (tid == 0) {
-            const double count = (double)S.cnt[0], cleft = (double)S.cnt[1];
+            const double count = (double)cnt[0], cleft = (double)cnt[1];
             double w = S.w[k], u = S.u[k], vel0 = S.v[k];
             const double collected = S.cr[k] - S.crb[k];
             const double ipriv = P.F_N * S.nov[k] + P.F_R * collected;
@@ -377,14 +594,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             double dvel = 0.0, dth = 0.0;
             const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
             auto walk = [&]() {
-                double uu = NAN;
-                if (wbuf) {
-                    if (used + 2 <= ucount) uu = words_to_double(wbuf[used], wbuf[used + 1]);
-                    used += 2;
-                } else {
-                    if (used < ucount) uu = ubuf[used];
-                    ++used;
-                }
+                const double uu = S.uni[draws++];
                 dvel = P.exp_vel_max;
                 dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
                 over = -1; mode = ABM_MODE_EXPLORE;
@@ -412,14 +622,57 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             p.v = p.v + dvel;
             // prove_velocity (agent.py:612-620): only in explore mode
             if (get_mode(over, w, P.T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = P.exp_vel_max;
-            p.x = p.x + p.v * cos(p.theta);
-            p.y = p.y - p.v * sin(p.theta);
+            double sn, cs;
+            sincos(p.theta, &sn, &cs);
+            p.x = p.x + p.v * cs;
+            p.y = p.y - p.v * sn;
             p = reflect_walls(p, S.rad[k], P.pad, P.width, P.height);
             S.x[k] = p.x; S.y[k] = p.y; S.th[k] = p.theta; S.v[k] = p.v;
             S.w[k] = w; S.u[k] = u; S.over[k] = over; S.mode[k] = mode;
             S.crb[k] = S.cr[k];
-            st.I_priv[a0 + k] = ipriv;
+            S.nov[k] = ipriv;   // read only by agent k's decision: now its I_priv
         }
+        // ---- thread 0, after the move: agent k's class and record for agent k+1
+        if (tid == 0 && f1 < N) {
+            const double cfx = S.x[f1] + S.rad[f1], cfy = S.y[f1] + S.rad[f1];
+            const int c = cue_class(S, P, k, f1, cfx, cfy, S.pid[f1]);
+            int extra = -1;
+            if (needs_record(c, excl)) {
+                S.rec[k] = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1);
+                extra = k;
+            }
+            S.cnt[6] = extra;
+            S.cnt[2 * par] = 0; S.cnt[2 * par + 1] = 0;   // agent k+1's counts (last read by k-1)
+        }
+        // ---- wave 1: agent k+1's records but agent k's
+        if (f1 < N && tid >= rbase && tid < cbase) {
+            for (int w = tid - rbase; w < P.words; w += 64) reinterpret_cast<uint32_t*>(S.V)[w] = 0u;
+            const Focal f = S.focal[par];
+            for (int q = tid - rbase; q < lst_n[par]; q += 64) {
+                const int j = lst[par][q];
+                S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1);
+            }
+        }
+        // ---- waves 2, 3: agent k+2's classes but agent k's and k+1's
+        if (f2 < N && tid >= cbase)
+            classify(S, P, N, f2, k, f1, excl, clsb[par2], lst[par2], &lst_n[par2], tid - cbase, cstride);
+        __syncthreads();
+        if (f1 >= N) break;
+        // ---- exclusion, raster and counts of agent k+1; meanwhile wave 2
+        // adds agent k's class for agent k+2 and wave 3 its focal geometry
+        raster_list(S, R, excl, lst[par], lst_n[par], S.cnt[6]);
+        if (tid == 0) lst_n[par] = 0;   // rebuilt for agent k+3
+        if (tid == cbase && f2 < N) {   // agent k's class for agent k+2; its record comes with the others
+            const double gx = S.x[f2] + S.rad[f2], gy = S.y[f2] + S.rad[f2];
+            const int c2 = cue_class(S, P, k, f2, gx, gy, S.pid[f2]);
+            clsb[par2][k] = c2;
+            if (needs_record(c2, excl)) {
+                lst[par2][lst_n[par2]] = k;
+                lst_n[par2] += 1;
+            }
+        }
+        if (tid == nthr - 64 && f2 < N) S.focal[par2] = focal_of(S, f2);
+        count_field(S, P, R, S.cnt + 2 * par, bits_out, f1);
         __syncthreads();
     }
     for (int i = tid; i < N; i += nthr) {
@@ -427,8 +680,9 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         st.x[g] = S.x[i]; st.y[g] = S.y[i]; st.theta[g] = S.th[i]; st.v[g] = S.v[i];
         st.w[g] = S.w[i]; st.u[g] = S.u[i]; st.collected_r_before[g] = S.crb[i];
         st.mode[g] = S.mode[i]; st.overriding_mode[g] = S.over[i];
+        st.I_priv[g] = S.nov[i];
     }
-    if (tid == 0 && u_used) u_used[blockIdx.x] = used;
+    if (tid == 0 && u_used) u_used[blockIdx.x] = used0 + (words ? 2 : 1) * draws;
     if (tid == 0 && status) status[blockIdx.x] = ABM_WORLD_DONE;
 }
 
@@ -821,7 +1075,9 @@ int check_state(const abm_forage_state* st, const char* who) {
 int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
                        const uint32_t* words, const int64_t* offset, const int32_t* count,
                        int32_t* used, int32_t* status, uint32_t* field_bits, void* stream) {
-    const int nthr = b->max_agents > 128 ? 256 : (b->max_agents > 64 ? 128 : 64);
+    // four waves: the step kernel overlaps one agent's move (wave 0) with
+    // the next agent's records (wave 1) and classes (waves 2, 3)
+    const int nthr = 256;
     const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
     int dev_max = 0;
     if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
@@ -874,6 +1130,20 @@ int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forag
         d.exp_vel_max = r.exp_vel_max; d.exp_theta_min = r.exp_theta_min;
         d.exp_theta_max = r.exp_theta_max; d.reloc_theta_max = r.reloc_theta_max;
         d.exp_stop_ratio = r.exp_stop_ratio;
+        // soc_v_field[k] survives the FOV mask unless linspace_k < fov0 or
+        // > fov1 (agent.py:593-595); linspace is monotonic, so the kept
+        // indices form one range
+        int lo = r.n_rays, hi = -1, kept = 0;
+        for (int k = 0; k < r.n_rays; ++k) {
+            const double ph = linspace_at(k, r.n_rays, d.step);
+            if (ph < r.fov0 || ph > r.fov1) continue;
+            if (lo == r.n_rays) lo = k;
+            hi = k;
+            ++kept;
+        }
+        if (kept != (hi >= lo ? hi - lo + 1 : 0))
+            return set_error(ABM_ERR_INVALID, "forage replica %d: the FOV keeps no single ray range", b);
+        d.keep_lo = lo; d.keep_hi = hi;
         agents += r.n_agents;
         words += (int64_t)r.n_agents * d.words;
         max_n = std::max(max_n, r.n_agents);
