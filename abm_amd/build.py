@@ -77,7 +77,7 @@ def build(verbose: bool = False) -> Path:
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

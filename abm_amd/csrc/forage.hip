@@ -496,11 +496,15 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
                    const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
                    int32_t* __restrict__ u_used, int32_t* __restrict__ status,
-                   uint32_t* __restrict__ bits_out, int maxN, int maxR) {
+                   uint32_t* __restrict__ bits_out, const int32_t* __restrict__ order, int maxN,
+                   int maxR) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) double smem[];
-    if (status && status[blockIdx.x] != kWorldReady) return;   // uniform over the block
-    const FGRep P = reps[blockIdx.x];
+    // blocks take the replicas largest first, so the longest id-order chains
+    // start at once and the small replicas fill in behind them
+    const int rb = order[blockIdx.x];
+    if (status && status[rb] != kWorldReady) return;   // uniform over the block
+    const FGRep P = reps[rb];
     const int N = P.n_agents, R = P.n_rays, a0 = P.agent_offset;
     const int tid = threadIdx.x, nthr = blockDim.x;
     FGSmem S = fg_carve(smem, maxN, maxR);
@@ -519,16 +523,16 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // the step's uniforms, staged in LDS: agent i's draw is the i-th (at most
     // one per agent); doubles, or pairs of words after those the resource
     // loop used (u_used[b] on entry); NaN past the end of the buffer
-    const int ucount = u_cnt[blockIdx.x];
-    const int used0 = words ? u_used[blockIdx.x] : 0;
+    const int ucount = u_cnt[rb];
+    const int used0 = words ? u_used[rb] : 0;
     for (int i = tid; i < N; i += nthr) {
         double uu = NAN;
         if (words) {
-            const uint32_t* wb = words + u_off[blockIdx.x];
+            const uint32_t* wb = words + u_off[rb];
             const int at = used0 + 2 * i;
             if (at + 2 <= ucount) uu = words_to_double(wb[at], wb[at + 1]);
         } else if (i < ucount) {
-            uu = uniforms[u_off[blockIdx.x] + i];
+            uu = uniforms[u_off[rb] + i];
         }
         S.uni[i] = uu;
     }
@@ -682,8 +686,8 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         st.mode[g] = S.mode[i]; st.overriding_mode[g] = S.over[i];
         st.I_priv[g] = S.nov[i];
     }
-    if (tid == 0 && u_used) u_used[blockIdx.x] = used0 + (words ? 2 : 1) * draws;
-    if (tid == 0 && status) status[blockIdx.x] = ABM_WORLD_DONE;
+    if (tid == 0 && u_used) u_used[rb] = used0 + (words ? 2 : 1) * draws;
+    if (tid == 0 && status) status[rb] = ABM_WORLD_DONE;
 }
 
 // ------------------------------------------------ agent-resource interaction
@@ -1094,7 +1098,7 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     }
     hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
                        (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
-                       field_bits, b->max_agents, b->max_rays);
+                       field_bits, (const int32_t*)b->order_dev, b->max_agents, b->max_rays);
     return check_hip(hipGetLastError(), "forage_step_kernel launch");
 }
 
@@ -1156,6 +1160,18 @@ int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forag
     if (!rc) rc = check_hip(hipMalloc(&bt->reps_dev, sizeof(FGRep) * n), "hipMalloc(forage replicas)");
     if (!rc) rc = check_hip(hipMemcpy(bt->reps_dev, host.data(), sizeof(FGRep) * n, hipMemcpyHostToDevice),
                            "upload forage replicas");
+    // update order of the blocks: largest replicas first (the id-order chain
+    // of a replica grows with its agents; rays add to each agent's step)
+    std::vector<int32_t> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        const double ca = (double)host[a].n_agents * (64.0 + host[a].n_rays / 32.0);
+        const double cc = (double)host[c].n_agents * (64.0 + host[c].n_rays / 32.0);
+        return ca > cc;
+    });
+    if (!rc) rc = check_hip(hipMalloc(&bt->order_dev, sizeof(int32_t) * n), "hipMalloc(forage order)");
+    if (!rc) rc = check_hip(hipMemcpy(bt->order_dev, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice),
+                           "upload forage order");
     if (rc) { abm_forage_batch_destroy(bt); return rc; }
     bt->agent_offset_host = new int32_t[n + 1];
     bt->field_offset_host = new int64_t[n + 1];
@@ -1172,6 +1188,7 @@ int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forag
 int abm_forage_batch_destroy(abm_forage_batch* b) {
     if (!b) return ABM_OK;
     if (b->reps_dev) (void)hipFree(b->reps_dev);
+    if (b->order_dev) (void)hipFree(b->order_dev);
     delete[] b->agent_offset_host;
     delete[] b->field_offset_host;
     delete b;

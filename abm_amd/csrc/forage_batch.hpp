@@ -14,6 +14,7 @@ struct abm_forage_batch {
     int32_t max_rays = 0;
     int device = 0;
     void* reps_dev = nullptr;
+    void* order_dev = nullptr;        // replicas by descending cost: the blocks' update order
     int32_t* agent_offset_host = nullptr;
     int64_t* field_offset_host = nullptr;
 };
