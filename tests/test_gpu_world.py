@@ -207,3 +207,62 @@ def test_world_rejects_bad_patch_count(gpu):
         ResourceWorld(sw, [WorldConfig(N_resc=65)])
     with pytest.raises(L.AbmError):
         ResourceWorld(sw, [WorldConfig(N_resc=2, patch_radius=120)])   # randint(30, -10)
+
+
+def test_world_free_run_collected_resource(gpu):
+    # C1 shape (N=10, radius 10, 10 patches of radius 30, quality U[0.5, 1],
+    # 500x500 walls, R=1200, the reference .env decision values) run free for
+    # 100 steps on 12 replicas, device and oracle from the same start and
+    # streams, no state reloaded.  Every replica's collected resource and
+    # RNG stream position must agree; should a replica's trajectory drift
+    # (atan2 of the bias may differ in the last ulp), the ensemble of
+    # collected resource must still agree (Welch t-test at 99.9 %).
+    from scipy import stats
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    p = fo.ForageParams(T_w=0.5, Eps_w=3.0, g_w=0.085, B_w=0.0, w_max=1.0, T_u=0.5, Eps_u=1.0,
+                        g_u=0.085, B_u=0.0, u_max=1.0, S_wu=0.25, S_uw=0.01, Tau=10, F_N=0.5,
+                        F_R=0.5, exp_vel_max=3.0, exp_theta_min=-0.5, exp_theta_max=0.5,
+                        reloc_theta_max=1.8, exp_stop_ratio=0.175)
+    B, N, R, W, steps = 12, 10, 1200, 500, 100
+    wk = dict(N_resc=10, patch_radius=30, min_resc_perpatch=200, max_resc_perpatch=1000,
+              min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+    worlds, allags, rngs, orngs = [], [], [], []
+    for b in range(B):
+        g = np.random.RandomState(4000 + b)
+        ags = []
+        for i in range(N):            # Simulation.create_agents (sims.py:526-532)
+            x = g.randint(30 - 10, W + 30 - 10)
+            y = g.randint(30 - 10, W + 30 - 10)
+            ags.append(fo.ForageAgent(i, 10, (x, y), g.uniform(0, 2 * PI), R, (-PI, PI), W, W, 30,
+                                      2000.0, False, True, p, pooling_time=0))
+        worlds.append(ags)
+        allags += ags
+        rngs.append(np.random.RandomState())
+        rngs[-1].set_state(g.get_state())
+        orngs.append(np.random.RandomState())
+        orngs[-1].set_state(g.get_state())
+    cfg = forage_config(N, W, R, p)
+    cfg.vision_range = 2000.0
+    sw = ForageSwarm([cfg] * B, [a.position[0] for a in allags], [a.position[1] for a in allags],
+                     [a.orientation for a in allags], radius=10, rngs=rngs)
+    rw = ResourceWorld(sw, [WorldConfig(**wk)] * B)
+    oracle = []
+    for b in range(B):
+        np.random.set_state(orngs[b].get_state())
+        oracle.append(oracle_world(worlds[b], W, wk))
+        orngs[b].set_state(np.random.get_state())
+    rw.create_resources()
+    for step in range(steps):
+        rw.step()
+        for b in range(B):
+            np.random.set_state(orngs[b].get_state())
+            oracle[b].step()
+            orngs[b].set_state(np.random.get_state())
+    got = sw.get_state("collected_r")["collected_r"].reshape(B, N).sum(axis=1)
+    exp = np.array([sum(a.collected_r for a in oracle[b].agents) for b in range(B)])
+    same = [b for b in range(B) if same_stream(rngs[b], orngs[b]) and got[b] == exp[b]]
+    print(f"collected resource: device {got.mean():.2f} +- {got.std():.2f}, oracle "
+          f"{exp.mean():.2f} +- {exp.std():.2f}; {len(same)} of {B} replicas identical", flush=True)
+    assert exp.sum() > 0
+    assert stats.ttest_ind(got, exp, equal_var=False).pvalue > 1e-3 or np.array_equal(got, exp)
+    assert len(same) >= B // 2
