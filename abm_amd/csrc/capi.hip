@@ -284,8 +284,13 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
     if (!rc) rc = check_hip(hipMalloc(&bt->tab32, sizeof(RayTab32) * ray_total), "hipMalloc(tab32)");
     if (!rc) rc = check_hip(hipMalloc(&bt->counters, sizeof(unsigned long long) * 4), "hipMalloc(counters)");
     if (!rc) rc = check_hip(hipMalloc(&bt->order_dev, sizeof(int32_t) * n), "hipMalloc(order)");
-    if (!rc && max_n > 2048)   // replicas too large for one CU's LDS take the wide step
+    if (!rc && max_n > 2048) {   // replicas too large for one CU's LDS take the wide step
         rc = check_hip(hipMalloc(&bt->wide_out, sizeof(float4) * (size_t)agents_total), "hipMalloc(wide)");
+        if (!rc) rc = check_hip(hipMalloc(&bt->grid_sorted, sizeof(float4) * (size_t)max_n), "hipMalloc(grid)");
+        if (!rc) rc = check_hip(hipMalloc(&bt->grid_start, sizeof(int32_t) * (kWideGridMaxCells + 1)),
+                                "hipMalloc(grid cells)");
+        if (!rc) rc = check_hip(hipMalloc(&bt->grid_param, 64), "hipMalloc(grid params)");
+    }
     if (!rc) {
         bt->tab_sin = bt->tab_cos + ray_total;
         bt->tab_dphi = bt->tab_cos + 2 * (size_t)ray_total;
@@ -328,6 +333,9 @@ int abm_vf_batch_destroy(abm_vf_batch* b) {
     if (b->counters) (void)hipFree(b->counters);
     if (b->order_dev) (void)hipFree(b->order_dev);
     if (b->wide_out) (void)hipFree(b->wide_out);
+    if (b->grid_sorted) (void)hipFree(b->grid_sorted);
+    if (b->grid_start) (void)hipFree(b->grid_start);
+    if (b->grid_param) (void)hipFree(b->grid_param);
     for (GroupPlan& gp : b->gplan) {
         if (gp.groups) (void)hipFree(gp.groups);
         if (gp.nib) (void)hipFree(gp.nib);

@@ -10,6 +10,20 @@
 
 namespace abm {
 
+// Uniform grid of the wide (large-flock) step over the periodic arena: cells
+// at least the visibility cut-off wide, so every pair that can be drawn lies
+// in the focal agent's 3 x 3 cells.
+constexpr int kWideGridMaxSide = 64;
+constexpr int kWideGridMaxCells = kWideGridMaxSide * kWideGridMaxSide;
+struct WideGrid {
+    int32_t ok;        // the grid path applies (torus, >= 3 cells a side, W/2 and H/2 beyond reach)
+    int32_t nx, ny;    // cells a side
+    int32_t pad_;
+    float cw, ch;      // cell width, height
+    float x0, y0;      // arena origin (pad)
+    float W, H;
+};
+
 // One replica as the kernels read it (built and validated on the host by
 // abm_vf_batch_create from an abm_vf_replica).
 struct VFRep {
@@ -73,6 +87,9 @@ struct abm_vf_batch {
     unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks
     int32_t* order_dev = nullptr;   // replica launch order (largest work first)
     float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
+    float4* grid_sorted = nullptr;  // wide step: the replica's agents sorted by grid cell
+    int32_t* grid_start = nullptr;  // wide step: first sorted agent of each cell (+ end)
+    void* grid_param = nullptr;     // wide step: the grid of the current replica (WideGrid)
     abm::GroupPlan gplan[3];        // G = 1, 2, 4
     // host copies
     abm::VFRep* reps_host = nullptr;

@@ -281,8 +281,30 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
 // i.e. when d > size * cot(2 pi / R) -- with a relative guard of 2e-4, and
 // compacts the others; only those take the float32 decision.  New poses go
 // to a scratch buffer and are copied back by a second kernel.
-constexpr int kWideWaves = 4;
-constexpr int kWideTile = 256;
+constexpr int kWideWaves = 8;    // focal agents per block, sharing the ray table
+constexpr int kWideTile = 512;  // objects per LDS tile of the full scan (one per thread)
+constexpr int kWideQueue = 128;   // grid path: candidates queued per wave (tile + cand memory)
+static_assert(sizeof(float4) * kWideTile + sizeof(int32_t) * kWideTile * kWideWaves >=
+              sizeof(float4) * kWideQueue * kWideWaves, "grid queues reuse the tile memory");
+
+// Grid cell of a centre: the coordinate taken modulo the arena, clamped
+// into the grid (rounding at the far edge).
+__device__ inline int grid_cell(const WideGrid& G, float cx, float cy) {
+    float u = cx - G.x0, v = cy - G.y0;
+    u = u - G.W * floorf(u / G.W);
+    v = v - G.H * floorf(v / G.H);
+    const int ix = min(G.nx - 1, max(0, (int)(u / G.cw)));
+    const int iy = min(G.ny - 1, max(0, (int)(v / G.ch)));
+    return iy * G.nx + ix;
+}
+
+// LDS written by some lanes of a wave and read by others: in-order per
+// wave once the writes have completed.
+__device__ inline void wave_lds_sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
 
 struct WideSmem {
     RayTab32* tab;      // transposed for 64 lanes
@@ -296,15 +318,22 @@ __global__ void __launch_bounds__(64 * kWideWaves)
 vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
                         abm_vf_state_f32 st, float4* __restrict__ out_pose,
                         uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
-                        unsigned long long* __restrict__ counters, float cull_c) {
+                        unsigned long long* __restrict__ counters, float cull_c,
+                        const WideGrid* __restrict__ grid, const float4* __restrict__ sorted,
+                        const int32_t* __restrict__ cell_start) {
     extern __shared__ __align__(16) double smem_d[];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int k = blockIdx.x * kWideWaves + wave;
-    const bool active = k < C.N;
     const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
     const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
     const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    // Grid path (counters and verify mode keep the full scan): focal agents
+    // in cell order, so a block's waves read the same neighbourhoods
+    const WideGrid G = *grid;
+    const bool use_grid = G.ok && !count && !verify;
+    const int slot = blockIdx.x * kWideWaves + wave;
+    const bool active = slot < C.N;
+    const int k = (use_grid && active) ? __float_as_int(sorted[slot].w) : slot;
     const int maxW = (RPL * 64 + 31) >> 5;
     WideSmem S;
     {
@@ -330,8 +359,82 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     sincos_f32(fth, &sth, &cth);
     const float u1x = cth, u1y = -sth;
     unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    // Grid path: each wave takes its focal agent's 3 x 3 cells from the
+    // sorted agents (the next 64 loaded while the current 64 are tested),
+    // queues the pairs inside the cut-off and decides them 64 at a time.  No
+    // block barrier inside: the waves share only the ray table.
+    if (use_grid) {
+        __syncthreads();   // the ray table
+        if (active) {
+            float4* cq = reinterpret_cast<float4*>(S.tile) + wave * kWideQueue;
+            auto decide = [&](const float4 o) {
+                PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
+                if (p.tie) tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, rp, C);   // rare
+                if (!p.ok) {
+                    int es = 0, ee = 0, ed = 0;
+                    exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                    p.start = es; p.end = ee; p.drawn = ed != 0;
+                }
+                if (p.drawn) raster(F, p.start, p.end, C.R);
+            };
+            int nq = 0;
+            const int fc = grid_cell(G, fx + fr, fy + fr);
+            const int fix = fc % G.nx, fiy = fc / G.nx;
+            // segment t = 0..8 of the neighbourhood: cell (fix + t%3 - 1, fiy + t/3 - 1)
+            auto cell_of_seg = [&](int t) {
+                const int cy = (fiy + t / 3 - 1 + G.ny) % G.ny;
+                return cy * G.nx + (fix + t % 3 - 1 + G.nx) % G.nx;
+            };
+            int t = 0, pos = cell_start[cell_of_seg(0)], end = cell_start[cell_of_seg(0) + 1];
+            auto skip_empty = [&]() {
+                while (t < 9 && pos >= end) {
+                    if (++t < 9) {
+                        const int c = cell_of_seg(t);
+                        pos = cell_start[c];
+                        end = cell_start[c + 1];
+                    }
+                }
+            };
+            skip_empty();
+            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 nxt = (t < 9 && pos + lane < end) ? sorted[pos + lane] : zero;
+            while (t < 9) {
+                const bool valid = pos + lane < end;
+                const float4 o = nxt;
+                pos += 64;
+                skip_empty();
+                nxt = (t < 9 && pos + lane < end) ? sorted[pos + lane] : zero;   // next 64, in flight
+                bool cnd = false;
+                if (valid && __float_as_int(o.w) != k && !(o.x == fx && o.y == fy)) {
+                    float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
+                    const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+                    const bool near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
+                    if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+                    if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+                    const float lim = o.z * cull_c;
+                    cnd = near_wrap || fmaf(dy, dy, dx * dx) <= lim * lim;
+                }
+                const uint64_t m = __ballot(cnd);
+                if (cnd) cq[nq + __popcll(m & ((1ull << lane) - 1ull))] = o;
+                nq += __popcll(m);
+                if (nq >= 64) {
+                    wave_lds_sync();
+                    decide(cq[lane]);
+                    const int rest = nq - 64;
+                    const float4 tq = lane < rest ? cq[64 + lane] : o;
+                    wave_lds_sync();
+                    if (lane < rest) cq[lane] = tq;
+                    nq = rest;
+                    wave_lds_sync();
+                }
+            }
+            wave_lds_sync();
+            if (lane < nq) decide(cq[lane]);
+            wave_lds_sync();
+        }
+    }
 
-    for (int t0 = 0; t0 < C.N; t0 += kWideTile) {
+    for (int t0 = 0; t0 < (use_grid ? 0 : C.N); t0 += kWideTile) {
         __syncthreads();
         if (t0 + tid < C.N) {
             const int gj = C.a0 + t0 + tid;
@@ -399,7 +502,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
             if (p.drawn) raster(F, p.start, p.end, C.R);
         }
     }
-    __syncthreads();
+    if (!use_grid) __syncthreads();
     if (!active) {
         if (count) {
             if (n_pairs) atomicAdd(&counters[0], n_pairs);
@@ -429,7 +532,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
                 spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
             }
         }
-#pragma unroll
+#pragma unroll 8
         for (int i = 0; i < RPL; ++i) {
             const RayTab32 T = S.tab[i * 64 + lane];
             const bool vb = (cur >> i) & 1u;
@@ -474,6 +577,86 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     }
 }
 
+// One block: the replica's grid (cells no narrower than the largest
+// visibility cut-off r_max * cull_c, plus 0.1 % and 1 px) and its agents
+// sorted by cell (counting sort in LDS).  The order inside a cell is free:
+// the field is an OR of rasters.
+__global__ void __launch_bounds__(1024)
+wide_grid_kernel(const VFRep* __restrict__ rp, abm_vf_state_f32 st, float cull_c,
+                 float4* __restrict__ sorted, int32_t* __restrict__ cell_start,
+                 WideGrid* __restrict__ grid) {
+    __shared__ int cnt[kWideGridMaxCells + 1];
+    __shared__ int part[1024];
+    __shared__ int rmax;
+    __shared__ WideGrid G;
+    const RepConst C = rep_const(rp);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (tid == 0) rmax = 0;
+    __syncthreads();
+    int rm = 0;
+    for (int i = tid; i < C.N; i += nthr) rm = max(rm, (int)st.radius[C.a0 + i]);
+    atomicMax(&rmax, rm);
+    __syncthreads();
+    if (tid == 0) {
+        WideGrid g;
+        g.pad_ = 0;
+        g.x0 = C.pad; g.y0 = C.pad; g.W = C.fW; g.H = C.fH;
+        const float reach = (float)rmax * cull_c * 1.001f + 1.0f;
+        g.nx = min(kWideGridMaxSide, (int)floorf(C.fW / reach));
+        g.ny = min(kWideGridMaxSide, (int)floorf(C.fH / reach));
+        g.ok = C.torus && g.nx >= 3 && g.ny >= 3 && C.hw - C.tguard > reach && C.hh - C.tguard > reach;
+        g.cw = g.nx > 0 ? C.fW / (float)g.nx : 1.f;
+        g.ch = g.ny > 0 ? C.fH / (float)g.ny : 1.f;
+        G = g;
+        *grid = g;
+    }
+    __syncthreads();
+    if (!G.ok) return;
+    const int ncell = G.nx * G.ny;
+    for (int c = tid; c <= ncell; c += nthr) cnt[c] = 0;
+    __syncthreads();
+    for (int i = tid; i < C.N; i += nthr) {
+        const int g = C.a0 + i;
+        const float r = (float)st.radius[g];
+        atomicAdd(&cnt[grid_cell(G, st.x[g] + r, st.y[g] + r)], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: per-thread runs of 4 cells, then a
+    // Hillis-Steele scan of the run totals
+    const int per = (ncell + nthr - 1) / nthr;
+    int run = 0;
+    for (int q = 0; q < per; ++q) {
+        const int c = tid * per + q;
+        if (c < ncell) run += cnt[c];
+    }
+    part[tid] = run;
+    __syncthreads();
+    for (int o = 1; o < nthr; o <<= 1) {
+        const int v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int acc = part[tid] - run;
+    for (int q = 0; q < per; ++q) {
+        const int c = tid * per + q;
+        if (c < ncell) {
+            const int n = cnt[c];
+            cnt[c] = acc;          // now the cell's write cursor
+            cell_start[c] = acc;
+            acc += n;
+        }
+    }
+    if (tid == 0) cell_start[ncell] = C.N;
+    __syncthreads();
+    for (int i = tid; i < C.N; i += nthr) {
+        const int g = C.a0 + i;
+        const float x = st.x[g], y = st.y[g], r = (float)st.radius[g];
+        const int at = atomicAdd(&cnt[grid_cell(G, x + r, y + r)], 1);
+        sorted[at] = make_float4(x, y, r, __int_as_float(i));
+    }
+}
+
 __global__ void copy_back_kernel(const float4* __restrict__ out_pose, abm_vf_state_f32 st, int a0, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -506,8 +689,14 @@ int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, flo
         // visibility cut-off: floor(proj_size/2) >= 1 needs d <= size * cot(2 pi / R)
         const float cull_c = (float)((1.0 / std::tan(kTwoPi / h.n_rays)) * (1.0 + 2e-4));
         const int blocks = (h.n_agents + kWideWaves - 1) / kWideWaves;
+        WideGrid* grid = (WideGrid*)b->grid_param;
+        hipLaunchKernelGGL(wide_grid_kernel, dim3(1), dim3(1024), 0, s, b->reps_dev + rep, *st, cull_c,
+                           b->grid_sorted, b->grid_start, grid);
+        if (int rc = check_hip(hipGetLastError(), "wide_grid_kernel launch")) return rc;
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWideWaves), lds, s, b->reps_dev + rep,
-                           b->tab32, *st, b->wide_out, bits, diag, flags, b->counters, cull_c);
+                           b->tab32, *st, b->wide_out, bits, diag, flags, b->counters, cull_c,
+                           (const WideGrid*)grid, (const float4*)b->grid_sorted,
+                           (const int32_t*)b->grid_start);
         if (int rc = check_hip(hipGetLastError(), "vf_step_f32_wide_kernel launch")) return rc;
         hipLaunchKernelGGL(copy_back_kernel, dim3((h.n_agents + 255) / 256), dim3(256), 0, s,
                            (const float4*)b->wide_out, *st, h.agent_offset, h.n_agents);

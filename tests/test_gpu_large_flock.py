@@ -73,3 +73,32 @@ def test_wide_kernel_seam_pairs(gpu):
     sw.step(order="synchronous", verify=True)
     pairs, rechecked, disagree = sw.fallback_counts()
     assert pairs == N * (N - 1) and disagree == 0
+
+
+@pytest.mark.parametrize("N,R,side,r,seam", [(8192, 2048, 4000.0, 2, False), (2304, 1024, 1500.0, 3, True),
+                                             (4096, 512, 3000.0, 5, False)])
+def test_wide_grid_path_matches_full_scan(gpu, N, R, side, r, seam):
+    # without counters or verify mode the wide step takes only the focal
+    # agent's 3 x 3 grid cells (cells wider than the cut-off); fields and
+    # poses must equal the full scan's (verify mode) bit for bit, here over
+    # three steps, also with agents crowded on the periodic seams (3 cells)
+    from abm_amd.vf import VFConfig, VFSwarm
+    rng = np.random.RandomState(5 + N)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    if seam:
+        lo, hi = 25.0, 25.0 + side
+        x = np.where(rng.rand(N) < 0.5, rng.uniform(lo - 4, lo + 4, N), rng.uniform(hi - 4, hi + 4, N))
+    else:
+        x = rng.uniform(30, 30 + side, N)
+    y = rng.uniform(30, 30 + side, N)
+    th, v = rng.uniform(0, 2 * PI, N), rng.uniform(0, 1.5, N)
+    cfg = VFConfig(n_agents=N, n_rays=R, width=side, height=side, boundary="infinite", ALP0=1.0, BET0=0.5)
+    a = VFSwarm([cfg], f32(x), f32(y), f32(th), f32(v), radius=r, precision="f32")
+    b = VFSwarm([cfg], f32(x), f32(y), f32(th), f32(v), radius=r, precision="f32")
+    for step in range(3):
+        a.step(order="synchronous", fields=True, verify=True)
+        b.step(order="synchronous", fields=True)
+        assert np.array_equal(a.fields_numpy(0), b.fields_numpy(0)), step
+        for p, q in zip(a.state_numpy(), b.state_numpy()):
+            assert np.array_equal(p, q), step
+    assert a.fallback_counts()[2] == 0
