@@ -298,6 +298,20 @@ __device__ inline int grid_cell(const WideGrid& G, float cx, float cy) {
     return iy * G.nx + ix;
 }
 
+// Decision and raster of one pair of the grid path (no counters, no verify).
+__device__ __forceinline__ void wide_decide(const float4 o, float u1x, float u1y, float fx, float fy,
+                                            float fth, float fr, const VFRep* rp, const RepConst& C,
+                                            uint32_t* F) {
+    PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
+    if (p.tie) tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, rp, C);   // rare
+    if (!p.ok) {
+        int es = 0, ee = 0, ed = 0;
+        exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+        p.start = es; p.end = ee; p.drawn = ed != 0;
+    }
+    if (p.drawn) raster(F, p.start, p.end, C.R);
+}
+
 // LDS written by some lanes of a wave and read by others: in-order per
 // wave once the writes have completed.
 __device__ inline void wave_lds_sync() {
@@ -348,9 +362,9 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     const int tab_off = rp->table_offset;
     for (int e = tid; e < RPL * 64; e += 64 * kWideWaves) {
         const int r = (e % 64) * RPL + e / 64;
-        RayTab32 z;
-        z.wcos = z.wsin = z.cos = z.sin = 0.f;
-        S.tab[e] = r < C.R ? tab32[tab_off + r] : z;
+        RayTab32 z = tab32[tab_off + (r < C.R ? r : 0)];
+        if (r >= C.R) z.wcos = z.wsin = z.cos = z.sin = 0.f;
+        S.tab[e] = z;
     }
     for (int w = lane; w < maxW; w += 64) F[w] = 0u;
     const int g = C.a0 + (active ? k : 0);
@@ -367,69 +381,53 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
         __syncthreads();   // the ray table
         if (active) {
             float4* cq = reinterpret_cast<float4*>(S.tile) + wave * kWideQueue;
-            auto decide = [&](const float4 o) {
-                PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
-                if (p.tie) tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, rp, C);   // rare
-                if (!p.ok) {
-                    int es = 0, ee = 0, ed = 0;
-                    exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
-                    p.start = es; p.end = ee; p.drawn = ed != 0;
-                }
-                if (p.drawn) raster(F, p.start, p.end, C.R);
-            };
             int nq = 0;
             const int fc = grid_cell(G, fx + fr, fy + fr);
-            const int fix = fc % G.nx, fiy = fc / G.nx;
-            // segment t = 0..8 of the neighbourhood: cell (fix + t%3 - 1, fiy + t/3 - 1)
-            auto cell_of_seg = [&](int t) {
-                const int cy = (fiy + t / 3 - 1 + G.ny) % G.ny;
-                return cy * G.nx + (fix + t % 3 - 1 + G.nx) % G.nx;
-            };
-            int t = 0, pos = cell_start[cell_of_seg(0)], end = cell_start[cell_of_seg(0) + 1];
-            auto skip_empty = [&]() {
-                while (t < 9 && pos >= end) {
-                    if (++t < 9) {
-                        const int c = cell_of_seg(t);
-                        pos = cell_start[c];
-                        end = cell_start[c + 1];
+            const int fiy = fc / G.nx, fix = fc - fiy * G.nx;
+            const int xl = fix == 0 ? G.nx - 1 : fix - 1, xr = fix == G.nx - 1 ? 0 : fix + 1;
+            const int yl = fiy == 0 ? G.ny - 1 : fiy - 1, yr = fiy == G.ny - 1 ? 0 : fiy + 1;
+            for (int t = 0; t < 9; ++t) {   // the 3 x 3 cells, wrapped
+                const int ty = t >= 6 ? 2 : (t >= 3 ? 1 : 0), tx = t - 3 * ty;
+                const int cy = ty == 0 ? yl : (ty == 1 ? fiy : yr);
+                const int cx = tx == 0 ? xl : (tx == 1 ? fix : xr);
+                const int c = cy * G.nx + cx;
+                const int end = cell_start[c + 1];
+                int pos = cell_start[c];
+                // loads always from a valid index; the next 64 are in flight
+                // while these are tested
+                float4 nxt = sorted[pos + lane < end ? pos + lane : 0];
+                while (pos < end) {
+                    const bool valid = pos + lane < end;
+                    const float4 o = nxt;
+                    pos += 64;
+                    nxt = sorted[pos + lane < end ? pos + lane : 0];
+                    bool cnd = false;
+                    if (valid && __float_as_int(o.w) != k && !(o.x == fx && o.y == fy)) {
+                        float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
+                        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+                        const bool near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
+                        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+                        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+                        const float lim = o.z * cull_c;
+                        cnd = near_wrap || fmaf(dy, dy, dx * dx) <= lim * lim;
                     }
-                }
-            };
-            skip_empty();
-            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 nxt = (t < 9 && pos + lane < end) ? sorted[pos + lane] : zero;
-            while (t < 9) {
-                const bool valid = pos + lane < end;
-                const float4 o = nxt;
-                pos += 64;
-                skip_empty();
-                nxt = (t < 9 && pos + lane < end) ? sorted[pos + lane] : zero;   // next 64, in flight
-                bool cnd = false;
-                if (valid && __float_as_int(o.w) != k && !(o.x == fx && o.y == fy)) {
-                    float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
-                    const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
-                    const bool near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
-                    if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
-                    if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
-                    const float lim = o.z * cull_c;
-                    cnd = near_wrap || fmaf(dy, dy, dx * dx) <= lim * lim;
-                }
-                const uint64_t m = __ballot(cnd);
-                if (cnd) cq[nq + __popcll(m & ((1ull << lane) - 1ull))] = o;
-                nq += __popcll(m);
-                if (nq >= 64) {
-                    wave_lds_sync();
-                    decide(cq[lane]);
-                    const int rest = nq - 64;
-                    const float4 tq = lane < rest ? cq[64 + lane] : o;
-                    wave_lds_sync();
-                    if (lane < rest) cq[lane] = tq;
-                    nq = rest;
-                    wave_lds_sync();
+                    const uint64_t m = __ballot(cnd);
+                    if (cnd) cq[nq + __popcll(m & ((1ull << lane) - 1ull))] = o;
+                    nq += __popcll(m);
+                    if (nq >= 64) {
+                        wave_lds_sync();
+                        wide_decide(cq[lane], u1x, u1y, fx, fy, fth, fr, rp, C, F);
+                        const int rest = nq - 64;
+                        const float4 tq = cq[64 + lane];   // lanes >= rest read a stale entry, unused
+                        wave_lds_sync();
+                        if (lane < rest) cq[lane] = tq;
+                        nq = rest;
+                        wave_lds_sync();
+                    }
                 }
             }
             wave_lds_sync();
-            if (lane < nq) decide(cq[lane]);
+            if (lane < nq) wide_decide(cq[lane], u1x, u1y, fx, fy, fth, fr, rp, C, F);
             wave_lds_sync();
         }
     }
@@ -593,8 +591,19 @@ wide_grid_kernel(const VFRep* __restrict__ rp, abm_vf_state_f32 st, float cull_c
     const int tid = threadIdx.x, nthr = blockDim.x;
     if (tid == 0) rmax = 0;
     __syncthreads();
+    // eight agents per thread per pass, their loads issued together
+    constexpr int kPer = 8;
     int rm = 0;
-    for (int i = tid; i < C.N; i += nthr) rm = max(rm, (int)st.radius[C.a0 + i]);
+    for (int base = 0; base < C.N; base += nthr * kPer) {
+        int rv[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = base + q * nthr + tid;
+            rv[q] = i < C.N ? (int)st.radius[C.a0 + i] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) rm = max(rm, rv[q]);
+    }
     atomicMax(&rmax, rm);
     __syncthreads();
     if (tid == 0) {
@@ -615,10 +624,17 @@ wide_grid_kernel(const VFRep* __restrict__ rp, abm_vf_state_f32 st, float cull_c
     const int ncell = G.nx * G.ny;
     for (int c = tid; c <= ncell; c += nthr) cnt[c] = 0;
     __syncthreads();
-    for (int i = tid; i < C.N; i += nthr) {
-        const int g = C.a0 + i;
-        const float r = (float)st.radius[g];
-        atomicAdd(&cnt[grid_cell(G, st.x[g] + r, st.y[g] + r)], 1);
+    for (int base = 0; base < C.N; base += nthr * kPer) {
+        float px[kPer], py[kPer], pr[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = base + q * nthr + tid;
+            const int g = C.a0 + (i < C.N ? i : 0);
+            px[q] = st.x[g]; py[q] = st.y[g]; pr[q] = (float)st.radius[g];
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (base + q * nthr + tid < C.N) atomicAdd(&cnt[grid_cell(G, px[q] + pr[q], py[q] + pr[q])], 1);
     }
     __syncthreads();
     // exclusive scan of the counts: per-thread runs of 4 cells, then a
@@ -649,11 +665,22 @@ wide_grid_kernel(const VFRep* __restrict__ rp, abm_vf_state_f32 st, float cull_c
     }
     if (tid == 0) cell_start[ncell] = C.N;
     __syncthreads();
-    for (int i = tid; i < C.N; i += nthr) {
-        const int g = C.a0 + i;
-        const float x = st.x[g], y = st.y[g], r = (float)st.radius[g];
-        const int at = atomicAdd(&cnt[grid_cell(G, x + r, y + r)], 1);
-        sorted[at] = make_float4(x, y, r, __int_as_float(i));
+    for (int base = 0; base < C.N; base += nthr * kPer) {
+        float px[kPer], py[kPer], pr[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = base + q * nthr + tid;
+            const int g = C.a0 + (i < C.N ? i : 0);
+            px[q] = st.x[g]; py[q] = st.y[g]; pr[q] = (float)st.radius[g];
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = base + q * nthr + tid;
+            if (i < C.N) {
+                const int at = atomicAdd(&cnt[grid_cell(G, px[q] + pr[q], py[q] + pr[q])], 1);
+                sorted[at] = make_float4(px[q], py[q], pr[q], __int_as_float(i));
+            }
+        }
     }
 }
 
