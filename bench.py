@@ -215,6 +215,8 @@ def other_workload(args):
               pair_ray_evals_per_s=Nl * (Nl - 1) * Rl / (ms * 1e-3))
     elif args.workload == "cs":
         return cs_workload(args)
+    elif args.workload == "c1":
+        return c1_workload(args)
     else:   # mixed ragged batch: even replicas forage, odd replicas flock
         rng = np.random.RandomState(20000)
         vf_cfgs, fg_cfgs, vx, fx_ = [], [], [], []
@@ -324,6 +326,82 @@ def cs_workload(args):
           note="agent-steps/s counts focal agents whose two projections were computed")
 
 
+C1_ENV = dict(T_w=0.5, Eps_w=3.0, g_w=0.085, B_w=0.0, w_max=1.0, T_u=0.5, Eps_u=1.0, g_u=0.085,
+              B_u=0.0, u_max=1.0, S_wu=0.25, S_uw=0.01, Tau=10, F_N=0.5, F_R=0.5, exp_vel_max=3.0,
+              exp_theta_min=-0.5, exp_theta_max=0.5, reloc_theta_max=1.8, exp_stop_ratio=0.175)
+C1_WORLD = dict(N_resc=10, patch_radius=30, min_resc_perpatch=200, max_resc_perpatch=1000,
+                min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+
+
+def _c1_start(seed, N=10, W=500):
+    """Simulation.create_agents of the C1 case (sims.py:526-532): positions
+    from randint, orientations from uniform, on the replica's own stream."""
+    g = np.random.RandomState(seed)
+    x, y, th = [], [], []
+    for _ in range(N):
+        x.append(float(g.randint(PAD - 10, W + PAD - 10)))
+        y.append(float(g.randint(PAD - 10, W + PAD - 10)))
+        th.append(g.uniform(0, 2 * np.pi))
+    return g, x, y, th
+
+
+def c1_workload(args):
+    """The reference's own CPU case (C1): collective foraging, N=10 agents of
+    radius 10, 10 patches (radius 30, quality U[0.5, 1]) in a 500x500
+    reflective arena, R=1200 over 2 pi, the .env decision values; the whole
+    loop body (resources, regeneration, Agent.update) per step on the device,
+    one replica as the reference runs it and 1000 seeds in one batch.  The
+    oracle's ForageWorld (numpy, one core) is timed first on the same case."""
+    from oracle import forage_oracle as fo
+    N, R, W = 10, 1200, 500
+    p = fo.ForageParams(**C1_ENV)
+    g, x, y, th = _c1_start(42)
+    ags = [fo.ForageAgent(i, 10, (x[i], y[i]), th[i], R, (-np.pi, np.pi), W, W, PAD, 2000.0, False,
+                          True, p, pooling_time=0) for i in range(N)]
+    np.random.set_state(g.get_state())
+    world = fo.ForageWorld(ags, W, W, PAD, 10, 30, 200, 1000, 0.5, 1.0, regenerate=True,
+                           teleport_exploit=False)
+    world.step()
+    t0, n_cpu = time.perf_counter(), 0
+    while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+        world.step()
+        n_cpu += 1
+    cpu_rate = n_cpu * N / (time.perf_counter() - t0)
+
+    import torch
+    from abm_amd.forage import ForageConfig, ForageSwarm, ResourceWorld, WorldConfig
+    cfg = ForageConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=PAD, vision_range=2000.0,
+                       **{k: v for k, v in C1_ENV.items()})
+
+    def run(n_rep, steps, warmup):
+        starts = [_c1_start(42 + b) for b in range(n_rep)]
+        sw = ForageSwarm([cfg] * n_rep, sum((s[1] for s in starts), []), sum((s[2] for s in starts), []),
+                         sum((s[3] for s in starts), []), radius=10, rngs=[s[0] for s in starts])
+        rw = ResourceWorld(sw, [WorldConfig(**C1_WORLD)] * n_rep)
+        rw.create_resources()
+        for _ in range(warmup):
+            rw.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rw.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    k = max(20, min(args.steps, 1000))
+    ms1 = run(1, k, max(5, args.warmup // 10))
+    msb = run(1000, max(10, k // 10), 3)
+    _line(args, "C1 collective foraging: N=10, R=1200, 10 patches (radius 30, quality U[0.5, 1]), "
+                "500x500 walls; resource loop and Agent.update on the device, one replica",
+          N / (ms1 * 1e-3), ms1, "f64", {"replicas": 1, "agents_per_replica": N, "rays": R},
+          batched_1000_seeds={"agent_steps_per_s": 1000 * N / (msb * 1e-3), "ms_per_step_wall": msb},
+          cpu_baseline={"value": cpu_rate, "unit": "agent-steps/s", "cores": 1, "kind": "port",
+                        "sample": f"oracle ForageWorld (numpy restatement of the loop body), the same "
+                                  f"C1 world, {n_cpu} steps on one core"},
+          note="wall time per step, including the host's MT19937 word peek and skip and one "
+               "synchronising read per step")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,12 +411,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "single", "mixed", "large", "cs"],
+    ap.add_argument("--workload", default="sweep",
+                    choices=["sweep", "single", "mixed", "large", "cs", "c1"],
                     help="sweep: 4096 replicas x N=50 x R=512 (default, the headline); single: "
                          "one replica N=100, R=1024, 1000x1000 torus; mixed: 2000 ragged replicas, "
                          "half foraging, half flocking; large: one flock N=8192, R=2048, "
                          "4000x4000 torus, synchronous order; cs: cooperative-signalling "
-                         "perception of 1000 replicas x N=50 x R=1200")
+                         "perception of 1000 replicas x N=50 x R=1200; c1: the reference's CPU "
+                         "foraging case (N=10, R=1200, 10 patches) on the device, with the "
+                         "oracle timed on one core")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
