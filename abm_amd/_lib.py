@@ -160,6 +160,8 @@ _SIGS = {
     "abm_forage_world_get_offsets": (C.c_int, [_P, _P]),
     "abm_forage_world_run": (C.c_int, [_P, _I32, C.POINTER(ForageState), _P, C.POINTER(PatchState),
                                        _P, _P, _P, _P, _P, _P, _P]),
+    "abm_forage_world_run_dev": (C.c_int, [_P, _I32, C.POINTER(ForageState), _P, C.POINTER(PatchState),
+                                           _P, _P, _P, _P, _P]),
     "abm_cs_projection": (C.c_int, [C.POINTER(CSParams), _I32, _P, _P, _P, _P, _P, _I32, _P, _P]),
     "abm_ensemble_stats": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32, _D, _D, _P, _P]),
     "abm_log_record": (C.c_int, [C.POINTER(LogBuffers), C.c_int64, C.c_int64, _I32, _P, _P, _P, _P,

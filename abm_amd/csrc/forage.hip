@@ -139,6 +139,7 @@ struct FGSmem {
     int* lst;       // step kernel: obstacles needing records, two lists of maxN by parity
     int* cls2;      // step kernel: classes for the agent after next (cls holds the next one's)
     double* uni;    // step kernel: the step's uniforms (or pairs of words), staged
+    uint32_t* mtk;  // step kernel: the replica's generator key (device generators)
     uint8_t* V;
     int* cnt;       // step kernel: [2p] total, [2p+1] left half by parity; [4 + p] list sizes
     double* lastd;  // keep_distance: distance of the last processed obstacle
@@ -159,6 +160,7 @@ __device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
     s.lst = reinterpret_cast<int*>(p); p += sizeof(int) * 2 * maxN;
     s.cls2 = reinterpret_cast<int*>(p); p += sizeof(int) * maxN;
     s.cnt = reinterpret_cast<int*>(p); p += sizeof(int) * 8;
+    s.mtk = reinterpret_cast<uint32_t*>(p); p += sizeof(uint32_t) * 624;
     s.V = reinterpret_cast<uint8_t*>(p);
     return s;
 }
@@ -167,7 +169,7 @@ static_assert(sizeof(Focal) <= 64, "two Focal slots of 64 bytes");
 
 size_t fg_lds_bytes(int maxN, int maxR) {
     return 128 + sizeof(double) * ((size_t)11 * maxN + 2) + sizeof(Rec) * (size_t)maxN +
-           sizeof(int) * ((size_t)9 * maxN + 8) + (size_t)maxR + 16;
+           sizeof(int) * ((size_t)9 * maxN + 8) + sizeof(uint32_t) * 624 + (size_t)maxR + 16;
 }
 
 // Records of one focal agent against the obstacle set marked in S.cls
@@ -478,6 +480,70 @@ __device__ void count_field(const FGSmem& S, const FGRep& P, int R, int* cnt, ui
     }
 }
 
+// ---- numpy's legacy MT19937 on the device (the published generator of
+// Matsumoto and Nishimura, in numpy's mt19937_state layout: 624 key words and
+// a position; numpy twists lazily, when a word is drawn at position 624)
+constexpr int kMtN = 624, kMtM = 397;
+
+__device__ inline uint32_t mt_temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+__device__ inline uint32_t mt_twist_word(uint32_t cur, uint32_t nxt, uint32_t far) {
+    const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
+    return far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+
+// key[i] for i = 0..623 in order, in place, by one thread.
+__device__ inline void mt_twist_serial(uint32_t* key) {
+    int i = 0;
+    for (; i < kMtN - kMtM; ++i) key[i] = mt_twist_word(key[i], key[i + 1], key[i + kMtM]);
+    for (; i < kMtN - 1; ++i) key[i] = mt_twist_word(key[i], key[i + 1], key[i + kMtM - kMtN]);
+    key[kMtN - 1] = mt_twist_word(key[kMtN - 1], key[0], key[kMtM - 1]);
+}
+
+// The same by a block of at least 227 threads: in the chunks [0, 227),
+// [227, 454), [454, 623) and {623} every element reads key[i + 1] not yet
+// rewritten and key[(i + 397) % 624] already final, so each chunk is read
+// in full, then written.  Every thread of the block calls it.
+__device__ inline void mt_twist_block(uint32_t* key) {
+    const int tid = threadIdx.x;
+    constexpr int lo[4] = {0, kMtN - kMtM, 2 * (kMtN - kMtM), kMtN - 1};
+    constexpr int hi[4] = {kMtN - kMtM, 2 * (kMtN - kMtM), kMtN - 1, kMtN};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int i = lo[c] + tid;
+        const bool mine = i < hi[c];
+        uint32_t v = 0u;
+        if (mine) {
+            const int far = i + kMtM < kMtN ? i + kMtM : i + kMtM - kMtN;
+            v = mt_twist_word(key[i], key[i + 1 < kMtN ? i + 1 : 0], key[far]);
+        }
+        __syncthreads();
+        if (mine) key[i] = v;
+        __syncthreads();
+    }
+}
+
+// Draws from a generator state held in LDS by one thread; never runs out.
+struct MtCursor {
+    uint32_t* key;
+    int pos, used;
+    bool out;
+    __device__ uint32_t next() {
+        if (pos >= kMtN) {
+            mt_twist_serial(key);
+            pos = 0;
+        }
+        ++used;
+        return mt_temper(key[pos++]);
+    }
+};
+
 // Status of a replica between the interaction kernel and the update.
 constexpr int kWorldReady = 3;
 
@@ -496,8 +562,8 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
                    const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
                    int32_t* __restrict__ u_used, int32_t* __restrict__ status,
-                   uint32_t* __restrict__ bits_out, const int32_t* __restrict__ order, int maxN,
-                   int maxR) {
+                   uint32_t* __restrict__ bits_out, const int32_t* __restrict__ order,
+                   abm_mt19937_state* __restrict__ mt, int maxN, int maxR) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) double smem[];
     // blocks take the replicas largest first, so the longest id-order chains
@@ -523,18 +589,44 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // the step's uniforms, staged in LDS: agent i's draw is the i-th (at most
     // one per agent); doubles, or pairs of words after those the resource
     // loop used (u_used[b] on entry); NaN past the end of the buffer
-    const int ucount = u_cnt[rb];
-    const int used0 = words ? u_used[rb] : 0;
-    for (int i = tid; i < N; i += nthr) {
-        double uu = NAN;
-        if (words) {
-            const uint32_t* wb = words + u_off[rb];
-            const int at = used0 + 2 * i;
-            if (at + 2 <= ucount) uu = words_to_double(wb[at], wb[at + 1]);
-        } else if (i < ucount) {
-            uu = uniforms[u_off[rb] + i];
+    const int ucount = mt ? 0 : u_cnt[rb];
+    const int used0 = (words || mt) ? u_used[rb] : 0;
+    int mt_pos0 = 0;
+    if (mt) {
+        // the replica's generator on the device: its next 2N words (a copy of
+        // the key twisted in LDS as needed), as N uniforms
+        for (int i = tid; i < kMtN; i += nthr) S.mtk[i] = mt[rb].key[i];
+        mt_pos0 = mt[rb].pos;
+        __syncthreads();
+        uint32_t* wb = reinterpret_cast<uint32_t*>(S.uni);
+        int made = 0, p = mt_pos0;
+        while (made < 2 * N) {
+            if (p >= kMtN) {
+                mt_twist_block(S.mtk);
+                p = 0;
+            }
+            const int take = min(kMtN - p, 2 * N - made);
+            for (int j = tid; j < take; j += nthr) wb[made + j] = mt_temper(S.mtk[p + j]);
+            made += take;
+            p += take;
         }
-        S.uni[i] = uu;
+        __syncthreads();
+        for (int i = tid; i < N; i += nthr) {   // each thread rewrites its own 8 bytes
+            const uint32_t w0 = wb[2 * i], w1 = wb[2 * i + 1];
+            S.uni[i] = words_to_double(w0, w1);
+        }
+    } else {
+        for (int i = tid; i < N; i += nthr) {
+            double uu = NAN;
+            if (words) {
+                const uint32_t* wb = words + u_off[rb];
+                const int at = used0 + 2 * i;
+                if (at + 2 <= ucount) uu = words_to_double(wb[at], wb[at + 1]);
+            } else if (i < ucount) {
+                uu = uniforms[u_off[rb] + i];
+            }
+            S.uni[i] = uu;
+        }
     }
     int draws = 0;   // meaningful in thread 0
     const int nl = R / 2;
@@ -686,8 +778,27 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         st.mode[g] = S.mode[i]; st.overriding_mode[g] = S.over[i];
         st.I_priv[g] = S.nov[i];
     }
-    if (tid == 0 && u_used) u_used[rb] = used0 + (words ? 2 : 1) * draws;
+    if (tid == 0 && u_used) u_used[rb] = used0 + (words || mt ? 2 : 1) * draws;
     if (tid == 0 && status) status[rb] = ABM_WORLD_DONE;
+    if (mt) {
+        // advance the generator by the 2 * draws words used: numpy's state
+        // after them is the key twisted v times at position pos0 + c - 624 v
+        if (tid == 0) S.cnt[7] = draws;
+        __syncthreads();
+        const int c = 2 * S.cnt[7];
+        int v = 0, pos = mt_pos0 + c;
+        if (pos > kMtN) {
+            v = (pos - 1) / kMtN;
+            pos -= kMtN * v;
+        }
+        if (v > 0) {
+            for (int i = tid; i < kMtN; i += nthr) S.mtk[i] = mt[rb].key[i];
+            __syncthreads();
+            for (int t = 0; t < v; ++t) mt_twist_block(S.mtk);
+            for (int i = tid; i < kMtN; i += nthr) mt[rb].key[i] = S.mtk[i];
+        }
+        if (tid == 0) mt[rb].pos = pos;
+    }
 }
 
 // ------------------------------------------------ agent-resource interaction
@@ -721,7 +832,8 @@ struct WordCursor {
 // next_uint32 & mask until the value is <= hi - 1 - lo, mask being the
 // smallest all-ones word covering it (random_bounded_uint64_fill with
 // use_masked; no draw when the range holds one value).
-__device__ inline int64_t np_randint(WordCursor& c, int64_t lo, int64_t hi) {
+template <typename Cursor>
+__device__ inline int64_t np_randint(Cursor& c, int64_t lo, int64_t hi) {
     const uint32_t rng = (uint32_t)(hi - 1 - lo);
     if (rng == 0u) return lo;
     uint32_t mask = rng;
@@ -732,7 +844,8 @@ __device__ inline int64_t np_randint(WordCursor& c, int64_t lo, int64_t hi) {
 }
 
 // RandomState.uniform(lo, hi) = lo + (hi - lo) * random_sample.
-__device__ inline double np_uniform(WordCursor& c, double lo, double hi) {
+template <typename Cursor>
+__device__ inline double np_uniform(Cursor& c, double lo, double hi) {
 #pragma clang fp contract(off)
     const uint32_t w0 = c.next(), w1 = c.next();
     return lo + (hi - lo) * words_to_double(w0, w1);
@@ -786,9 +899,10 @@ struct FreshPatch {
 // this step, in kill order, or create_resources' new patches) from the word
 // buffer, then compact the list: live patches in order, new ones appended.
 // One thread.  Commits nothing unless every placement fits in the buffer.
+template <typename Cursor>
 __device__ int place_patches(const WRep& W, abm_patch_state ps, int b, int kind, int n_pending,
                              const int32_t* __restrict__ pend_id, int32_t* __restrict__ kill_rank,
-                             WordCursor& cur, FreshPatch* fresh) {
+                             Cursor& cur, FreshPatch* fresh) {
     const int64_t po = W.patch_offset;
     const int count = ps.count[b];
     int n_new = 0;
@@ -876,10 +990,13 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
                     int32_t* __restrict__ kill_rank, int32_t* __restrict__ pend,
                     int32_t* __restrict__ pend_id, const uint32_t* __restrict__ words,
                     const int64_t* __restrict__ w_off, const int32_t* __restrict__ w_cnt,
-                    int32_t* __restrict__ w_used, int32_t* __restrict__ status) {
+                    int32_t* __restrict__ w_used, int32_t* __restrict__ status,
+                    abm_mt19937_state* __restrict__ mt) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) uint64_t member[];   // [patch][64-agent chunk]
     __shared__ FreshPatch fresh[kMaxPatches];
+    __shared__ uint32_t mkey[kMtN];
+    __shared__ int s_kind, s_pending, s_pos;
     const int b = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
     const FGRep P = reps[b];
     const WRep W = wreps[b];
@@ -1004,6 +1121,48 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
         kind = kPendFill;
         n_pending = 0;
     }
+    if (mt) {
+        // ---- placements from the replica's generator on the device: its
+        // state in LDS for thread 0's draws, written back after
+        if (tid == 0) { s_kind = kind; s_pending = n_pending; }
+        __syncthreads();
+        kind = s_kind;
+        n_pending = s_pending;
+        const bool update_follows = kind != kPendFill;
+        if (kind == kPendNone) {
+            if (tid == 0) {
+                pend[2 * b] = kPendNone; pend[2 * b + 1] = 0;
+                w_used[b] = 0;
+                status[b] = kWorldReady;
+            }
+            return;
+        }
+        for (int i = tid; i < kMtN; i += nthr) mkey[i] = mt[b].key[i];
+        if (tid == 0) s_pos = mt[b].pos;
+        __syncthreads();
+        if (tid == 0) {
+            MtCursor cur{mkey, s_pos, 0, false};
+            const int rc = place_patches(W, ps, b, kind, n_pending, pend_id, kill_rank, cur, fresh);
+            w_used[b] = cur.used;
+            if (rc == ABM_WORLD_DONE) {
+                pend[2 * b] = kPendNone; pend[2 * b + 1] = 0;
+                status[b] = update_follows ? kWorldReady : ABM_WORLD_DONE;
+                s_pos = cur.pos;
+            } else {
+                // too many retries: the reference raises; nothing placed, the
+                // generator stays where it was
+                pend[2 * b] = kind; pend[2 * b + 1] = n_pending;
+                status[b] = rc;
+                s_pos = -1;
+            }
+        }
+        __syncthreads();
+        if (s_pos >= 0) {
+            for (int i = tid; i < kMtN; i += nthr) mt[b].key[i] = mkey[i];
+            if (tid == 0) mt[b].pos = s_pos;
+        }
+        return;
+    }
     if (tid != 0) return;
     // ---- placements: kept words for the random walk of the update that follows
     const bool update_follows = kind != kPendFill;
@@ -1078,7 +1237,8 @@ int check_state(const abm_forage_state* st, const char* who) {
 // Agent.update for every replica (all, or those a status array marks ready).
 int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
                        const uint32_t* words, const int64_t* offset, const int32_t* count,
-                       int32_t* used, int32_t* status, uint32_t* field_bits, void* stream) {
+                       int32_t* used, int32_t* status, uint32_t* field_bits, void* stream,
+                       abm_mt19937_state* mt = nullptr) {
     // four waves: the step kernel overlaps one agent's move (wave 0) with
     // the next agent's records (wave 1) and classes (waves 2, 3)
     const int nthr = 256;
@@ -1098,7 +1258,7 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     }
     hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
                        (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
-                       field_bits, (const int32_t*)b->order_dev, b->max_agents, b->max_rays);
+                       field_bits, (const int32_t*)b->order_dev, mt, b->max_agents, b->max_rays);
     return check_hip(hipGetLastError(), "forage_step_kernel launch");
 }
 
@@ -1307,34 +1467,62 @@ int abm_forage_world_get_offsets(const abm_forage_world* w, int64_t* patch_offse
     return ABM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One phase of the resource loop; the placements and the walk draw from a
+// word buffer (words) or from device generators (mt).
+int world_run(abm_forage_world* w, int32_t phase, const abm_forage_state* st, int32_t* pool_success,
+              const abm_patch_state* ps, const uint32_t* words, const int64_t* word_offset,
+              const int32_t* word_count, int32_t* words_used, int32_t* status, uint32_t* field_bits,
+              void* stream, abm_mt19937_state* mt, const char* who) {
+    if (!w || !st || !ps) return set_error(ABM_ERR_INVALID, "%s: NULL world, state or patches", who);
+    if (int rc = check_state(st, who)) return rc;
+    if (!pool_success || !words_used || !status)
+        return set_error(ABM_ERR_INVALID, "%s: pool_success, words_used and status are required", who);
+    if (w->patch_total > 0 && (!ps->count || !ps->id || !ps->x || !ps->y || !ps->radius || !ps->units ||
+                               !ps->left || !ps->quality))
+        return set_error(ABM_ERR_INVALID, "%s: every patch array is required", who);
+    if (!ps->count) return set_error(ABM_ERR_INVALID, "%s: patch count array is required", who);
+    abm_forage_batch* b = w->batch;
+    const size_t lds = w->member_lds;
+    if (lds > 65536) return set_error(ABM_ERR_UNSUPPORTED, "%s: %zu B of membership masks", who, lds);
+    hipLaunchKernelGGL(forage_world_kernel, dim3(b->n_replicas), dim3(256), lds, (hipStream_t)stream,
+                       (const FGRep*)b->reps_dev, (const WRep*)w->wreps_dev, (int)phase, *st, pool_success,
+                       *ps, w->kill_rank, w->pend, w->pend_id, words, word_offset, word_count, words_used,
+                       status, mt);
+    if (int rc = check_hip(hipGetLastError(), "forage_world_kernel launch")) return rc;
+    if (phase == ABM_WORLD_FILL) return ABM_OK;
+    // the update draws its walk after the placements
+    return launch_forage_step(b, st, nullptr, words, word_offset, word_count, words_used, status,
+                              field_bits, stream, mt);
+}
+
+}  // namespace
+
+extern "C" {
+
 int abm_forage_world_run(abm_forage_world* w, int32_t phase, const abm_forage_state* st,
                          int32_t* pool_success, const abm_patch_state* ps, const uint32_t* words,
                          const int64_t* word_offset, const int32_t* word_count, int32_t* words_used,
                          int32_t* status, uint32_t* field_bits, void* stream) {
-    if (!w || !st || !ps) return set_error(ABM_ERR_INVALID, "abm_forage_world_run: NULL world, state or patches");
     if (phase != ABM_WORLD_STEP && phase != ABM_WORLD_RESUME && phase != ABM_WORLD_FILL)
         return set_error(ABM_ERR_INVALID, "abm_forage_world_run: unknown phase %d", phase);
-    if (int rc = check_state(st, "abm_forage_world_run")) return rc;
-    if (!pool_success || !words_used || !status)
-        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: pool_success, words_used and status are required");
-    if (w->patch_total > 0 && (!ps->count || !ps->id || !ps->x || !ps->y || !ps->radius || !ps->units ||
-                               !ps->left || !ps->quality))
-        return set_error(ABM_ERR_INVALID, "abm_forage_world_run: every patch array is required");
-    if (!ps->count) return set_error(ABM_ERR_INVALID, "abm_forage_world_run: patch count array is required");
     if (!words || !word_offset || !word_count)
         return set_error(ABM_ERR_INVALID, "abm_forage_world_run: the word buffer is required");
-    abm_forage_batch* b = w->batch;
-    const size_t lds = w->member_lds;
-    if (lds > 65536) return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_world_run: %zu B of membership masks", lds);
-    hipLaunchKernelGGL(forage_world_kernel, dim3(b->n_replicas), dim3(256), lds, (hipStream_t)stream,
-                       (const FGRep*)b->reps_dev, (const WRep*)w->wreps_dev, (int)phase, *st, pool_success,
-                       *ps, w->kill_rank, w->pend, w->pend_id, words, word_offset, word_count, words_used,
-                       status);
-    if (int rc = check_hip(hipGetLastError(), "forage_world_kernel launch")) return rc;
-    if (phase == ABM_WORLD_FILL) return ABM_OK;
-    // the update reads its walk's words after those the placements used
-    return launch_forage_step(b, st, nullptr, words, word_offset, word_count, words_used, status,
-                              field_bits, stream);
+    return world_run(w, phase, st, pool_success, ps, words, word_offset, word_count, words_used, status,
+                     field_bits, stream, nullptr, "abm_forage_world_run");
+}
+
+int abm_forage_world_run_dev(abm_forage_world* w, int32_t phase, const abm_forage_state* st,
+                             int32_t* pool_success, const abm_patch_state* ps, abm_mt19937_state* mt,
+                             int32_t* words_used, int32_t* status, uint32_t* field_bits, void* stream) {
+    if (phase != ABM_WORLD_STEP && phase != ABM_WORLD_FILL)
+        return set_error(ABM_ERR_INVALID, "abm_forage_world_run_dev: phase %d (STEP or FILL)", phase);
+    if (!mt) return set_error(ABM_ERR_INVALID, "abm_forage_world_run_dev: NULL generator states");
+    return world_run(w, phase, st, pool_success, ps, nullptr, nullptr, nullptr, words_used, status,
+                     field_bits, stream, mt, "abm_forage_world_run_dev");
 }
 
 int abm_forage_projection(int32_t R, double fov0, double fov1, int32_t visual_exclusion,

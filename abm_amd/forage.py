@@ -272,6 +272,8 @@ class WorldConfig:
 
 
 _PATCH_F64 = ("x", "y", "radius", "units", "left", "quality")
+_MT_WORDS = 625                       # abm_mt19937_state: 624 key words and the position
+_MT_BYTES = 4 * _MT_WORDS
 TIMEOUT_MESSAGE = "Reached timeout while trying to create resources without overlap!"
 
 
@@ -287,9 +289,17 @@ class ResourceWorld:
     words (patch placement draws randint and uniform from them, the random
     walk two per uniform) and advanced afterwards by exactly the words used.
     A replica whose placements need more than ``regen_words`` words is
-    resumed with a larger buffer; nothing is drawn twice."""
+    resumed with a larger buffer; nothing is drawn twice.
 
-    def __init__(self, swarm: ForageSwarm, configs: Sequence[WorldConfig], regen_words: int = 32):
+    With ``device_rng=True`` the generators' states move to device memory
+    instead and the kernels draw from them there: a step is two launches and
+    no host work or synchronisation.  The numpy generators are then stale
+    until ``sync_rng()`` writes the device states back into them (call it
+    before anything else draws from them); ``check()`` reads the replicas'
+    status and raises the reference's exception for a placement timeout."""
+
+    def __init__(self, swarm: ForageSwarm, configs: Sequence[WorldConfig], regen_words: int = 32,
+                 device_rng: bool = False):
         self.swarm, self.lib = swarm, swarm.lib
         self.configs = list(configs)
         nb = len(swarm.configs)
@@ -316,6 +326,39 @@ class ResourceWorld:
         self.regen_words = int(regen_words)
         self.resumes = 0      # phases resumed for want of words
         self._n = np.diff(swarm.agent_offset).astype(np.int64)
+        self.device_rng = bool(device_rng)
+        self._mt = None
+        if self.device_rng:
+            self._mt = torch.zeros(nb * _MT_WORDS, dtype=torch.int32, device=dev)
+            self.rng_to_device()
+
+    # ---------------------------------------------------------------- generators
+    def rng_to_device(self):
+        """Copy every replica's numpy generator state to the device."""
+        addrs = self.swarm._state_addrs()
+        host = np.empty((len(addrs), _MT_WORDS), dtype=np.uint32)
+        for b, a in enumerate(addrs):
+            C.memmove(host[b].ctypes.data, int(a), _MT_BYTES)
+        self._mt.copy_(torch.from_numpy(host.view(np.int32).reshape(-1)))
+
+    def sync_rng(self):
+        """Write the device generator states back into the numpy generators,
+        so they stand where the reference's would (after ``check()``)."""
+        if not self.device_rng:
+            return
+        self.check()
+        host = self._mt.cpu().numpy().view(np.uint32).reshape(-1, _MT_WORDS)
+        for b, a in enumerate(self.swarm._state_addrs()):
+            C.memmove(int(a), host[b].ctypes.data, _MT_BYTES)
+
+    def check(self):
+        """Raise for a replica whose placements timed out (the reference's
+        exception) or whose patch count is invalid."""
+        status = self._ret[1].cpu().numpy()
+        if np.any(status == L.WORLD_TIMEOUT) or np.any(status == L.WORLD_NEED_WORDS):
+            raise Exception(TIMEOUT_MESSAGE)
+        if np.any(status == L.WORLD_BAD_COUNT):
+            raise L.AbmError(L.ABM_ERR_INVALID, "a patch count is outside [0, N_resc]")
 
     def close(self):
         if getattr(self, "handle", None):
@@ -357,6 +400,8 @@ class ResourceWorld:
         self._run(L.WORLD_STEP, fields, stream)
 
     def _run(self, phase, fields, stream):
+        if self.device_rng:
+            return self._run_dev(phase, fields, stream)
         sw = self.swarm
         nb = len(sw.configs)
         reserve = np.zeros(nb, dtype=np.int64) if phase == L.WORLD_FILL else 2 * self._n
@@ -402,6 +447,17 @@ class ResourceWorld:
                         raise L.AbmError(L.ABM_ERR_UNSUPPORTED, "patch placement needs more than 2^24 words")
                     ph, active = L.WORLD_RESUME, need
                     self.resumes += 1
+
+    def _run_dev(self, phase, fields, stream):
+        sw = self.swarm
+        st = L.ForageState(**{k: sw.t[k].data_ptr() for k in _F64 + ("novelty",) + _I32})
+        ps = L.PatchState(**{k: self.p[k].data_ptr() for k in ("count", "id") + _PATCH_F64})
+        fb = L.ptr(sw.fields) if fields else None
+        with torch.cuda.device(sw.device):
+            s = stream if stream is not None else torch.cuda.current_stream()
+            L.check(self.lib.abm_forage_world_run_dev(
+                self.handle, phase, C.byref(st), L.ptr(self.pool_success), C.byref(ps), L.ptr(self._mt),
+                L.ptr(self._ret[0]), L.ptr(self._ret[1]), fb, L.stream_ptr(s)))
 
 
 # ------------------------------------------------------- reference API

@@ -241,13 +241,13 @@ def other_workload(args):
                           rngs=[np.random.RandomState(30000 + i) for i in range(len(fg_cfgs))])
         # each foraging replica runs the whole loop body: 10 patches of radius
         # 30, quality U[0.5, 1], 200..1000 units, regenerated when depleted
+        # (the generators on the device: no host work or synchronisation per step)
         world = ResourceWorld(fsw, [WorldConfig(N_resc=10, patch_radius=30, min_resc_quality=0.5,
                                                 max_resc_quality=1.0, teleport_exploit=False)
-                                    for _ in fg_cfgs])
+                                    for _ in fg_cfgs], device_rng=True)
         world.create_resources()
         vsw.autotune(trial_steps=2)
         ms_vf = time_steps(vsw, args.steps, args.warmup, not args.no_graph)[0]
-        # a foraging step reads back the words it used: eager launches
         for _ in range(min(args.warmup, 3)):
             world.step()
         torch.cuda.synchronize()
@@ -257,6 +257,7 @@ def other_workload(args):
             world.step()
         torch.cuda.synchronize()
         ms_fg = (time.perf_counter() - t0) * 1e3 / n_fs
+        world.sync_rng()
         n_vf = int(vsw.batch.info.n_agents_total)
         n_fg = int(fsw.info.n_agents_total)
         ms = ms_vf + ms_fg
@@ -350,7 +351,8 @@ def c1_workload(args):
     radius 10, 10 patches (radius 30, quality U[0.5, 1]) in a 500x500
     reflective arena, R=1200 over 2 pi, the .env decision values; the whole
     loop body (resources, regeneration, Agent.update) per step on the device,
-    one replica as the reference runs it and 1000 seeds in one batch.  The
+    one replica as the reference runs it and 1000 seeds in one batch, with the
+    generators on the device (and, for one replica, on the host).  The
     oracle's ForageWorld (numpy, one core) is timed first on the same case."""
     from oracle import forage_oracle as fo
     N, R, W = 10, 1200, 500
@@ -373,11 +375,11 @@ def c1_workload(args):
     cfg = ForageConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=PAD, vision_range=2000.0,
                        **{k: v for k, v in C1_ENV.items()})
 
-    def run(n_rep, steps, warmup):
+    def run(n_rep, steps, warmup, device_rng=True):
         starts = [_c1_start(42 + b) for b in range(n_rep)]
         sw = ForageSwarm([cfg] * n_rep, sum((s[1] for s in starts), []), sum((s[2] for s in starts), []),
                          sum((s[3] for s in starts), []), radius=10, rngs=[s[0] for s in starts])
-        rw = ResourceWorld(sw, [WorldConfig(**C1_WORLD)] * n_rep)
+        rw = ResourceWorld(sw, [WorldConfig(**C1_WORLD)] * n_rep, device_rng=device_rng)
         rw.create_resources()
         for _ in range(warmup):
             rw.step()
@@ -386,20 +388,25 @@ def c1_workload(args):
         for _ in range(steps):
             rw.step()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / steps
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        rw.sync_rng()
+        return ms
 
     k = max(20, min(args.steps, 1000))
     ms1 = run(1, k, max(5, args.warmup // 10))
+    ms1_host = run(1, k, max(5, args.warmup // 10), device_rng=False)
     msb = run(1000, max(10, k // 10), 3)
     _line(args, "C1 collective foraging: N=10, R=1200, 10 patches (radius 30, quality U[0.5, 1]), "
                 "500x500 walls; resource loop and Agent.update on the device, one replica",
           N / (ms1 * 1e-3), ms1, "f64", {"replicas": 1, "agents_per_replica": N, "rays": R},
           batched_1000_seeds={"agent_steps_per_s": 1000 * N / (msb * 1e-3), "ms_per_step_wall": msb},
+          host_generators={"agent_steps_per_s": N / (ms1_host * 1e-3), "ms_per_step_wall": ms1_host,
+                           "note": "numpy generators peeked and skipped on the host every step"},
           cpu_baseline={"value": cpu_rate, "unit": "agent-steps/s", "cores": 1, "kind": "port",
                         "sample": f"oracle ForageWorld (numpy restatement of the loop body), the same "
                                   f"C1 world, {n_cpu} steps on one core"},
-          note="wall time per step, including the host's MT19937 word peek and skip and one "
-               "synchronising read per step")
+          note="wall time per step; the replicas' MT19937 generators live on the device "
+               "(written back to numpy after the timed steps)")
 
 
 def main():

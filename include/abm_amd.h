@@ -352,6 +352,14 @@ int abm_forage_projection(int32_t n_rays, double fov0, double fov1, int32_t visu
                           const double* ox, const double* oy, int32_t n_social,
                           int32_t n_nonsocial, double* out_field, void* stream);
 
+/* numpy's legacy MT19937 state (numpy/random/src/mt19937/mt19937.h,
+ * mt19937_state): the address a RandomState's bit generator reports as
+ * ctypes.state_address points at exactly this struct. */
+typedef struct abm_mt19937_state {
+    uint32_t key[624];
+    int32_t pos;
+} abm_mt19937_state;
+
 /* ------------------------------------------- agent-resource interaction */
 /* The resource patches of every foraging replica and the loop body around
  * Agent.update that uses them (abm/simulation/sims.py:789-861 with agent-agent
@@ -426,14 +434,20 @@ int abm_forage_world_run(abm_forage_world* world, int32_t phase, const abm_forag
                          const int32_t* word_count, int32_t* words_used, int32_t* status,
                          uint32_t* field_bits, void* stream);
 
+/* The same loop body with each replica's numpy legacy MT19937 state kept in
+ * device memory (mt: one abm_mt19937_state per replica, the layout numpy
+ * keeps behind a RandomState): the placements draw from it in the interaction
+ * kernel and the random walk in the update, which advances it by exactly the
+ * words used, so no host peek, skip or synchronisation is needed per step and
+ * no replica ever waits for words.  Phases ABM_WORLD_STEP and ABM_WORLD_FILL.
+ * status[b] and words_used[b] (placements + walk) as above, read when the
+ * caller wishes; ABM_WORLD_TIMEOUT leaves the replica's generator untouched. */
+int abm_forage_world_run_dev(abm_forage_world* world, int32_t phase, const abm_forage_state* state,
+                             int32_t* pool_success, const abm_patch_state* patches,
+                             abm_mt19937_state* mt_states, int32_t* words_used, int32_t* status,
+                             uint32_t* field_bits, void* stream);
+
 /* ------------------------------------------------ host random numbers */
-/* numpy's legacy MT19937 state (numpy/random/src/mt19937/mt19937.h,
- * mt19937_state): the address a RandomState's bit generator reports as
- * ctypes.state_address points at exactly this struct. */
-typedef struct abm_mt19937_state {
-    uint32_t key[624];
-    int32_t pos;
-} abm_mt19937_state;
 
 /* Host functions (no GPU).  Replace the per-replica draws of
  * np.random.uniform in the random walk (abm/agent/supcalc.py:38-47):

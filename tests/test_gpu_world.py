@@ -125,8 +125,8 @@ def same_stream(a, b):
     return a.get_state()[2] == b.get_state()[2] and np.array_equal(a.get_state()[1], b.get_state()[1])
 
 
-@pytest.mark.parametrize("regen_words", [32, 1])
-def test_world_matches_oracle(gpu, regen_words):
+@pytest.mark.parametrize("regen_words,device_rng", [(32, False), (1, False), (32, True)])
+def test_world_matches_oracle(gpu, regen_words, device_rng):
     from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
     worlds, cfgs, wcfgs, allags = [], [], [], []
     for b, (N, W, R, wk, mixed) in enumerate(WORLDS):
@@ -139,15 +139,16 @@ def test_world_matches_oracle(gpu, regen_words):
     orngs = [np.random.RandomState(900 + b) for b in range(len(WORLDS))]
     sw = ForageSwarm(cfgs, [a.position[0] for a in allags], [a.position[1] for a in allags],
                      [a.orientation for a in allags], radius=10, rngs=rngs)
-    rw = ResourceWorld(sw, wcfgs, regen_words=regen_words)
+    rw = ResourceWorld(sw, wcfgs, regen_words=regen_words, device_rng=device_rng)
     oracle = []
     for b, (ags, W, wk) in enumerate(worlds):
         load_agents(sw, rw, ags, b)
         np.random.set_state(orngs[b].get_state())
         oracle.append(oracle_world(ags, W, wk))
         orngs[b].set_state(np.random.get_state())
-    print(f"create_resources regen_words={regen_words}", flush=True)
+    print(f"create_resources regen_words={regen_words} device_rng={device_rng}", flush=True)
     rw.create_resources()
+    rw.sync_rng()
     for b, world in enumerate(oracle):
         assert same_stream(rngs[b], orngs[b]), b
         compare(sw, rw, world, b, ("create", b))
@@ -155,6 +156,7 @@ def test_world_matches_oracle(gpu, regen_words):
     first = [[(p.id, p.position[0], p.position[1]) for p in w.patches] for w in oracle]
     for step in range(80):
         rw.step(fields=True)
+        rw.sync_rng()
         for b, world in enumerate(oracle):
             before = [(p.id, p.position[0], p.position[1]) for p in world.patches]
             np.random.set_state(orngs[b].get_state())
@@ -173,6 +175,8 @@ def test_world_matches_oracle(gpu, regen_words):
     assert sorted(p.id for p in oracle[0].patches) == sorted(i for i, _, _ in first[0])
     if regen_words == 1:
         assert rw.resumes > 0
+    if device_rng:
+        assert rw.resumes == 0
 
 
 def test_world_placement_timeout_raises(gpu):
@@ -190,6 +194,12 @@ def test_world_placement_timeout_raises(gpu):
     rw = ResourceWorld(sw, [WorldConfig(**wk)])
     with pytest.raises(Exception, match=TIMEOUT_MESSAGE):
         rw.create_resources()
+    # with the generator on the device the timeout shows when the status is read
+    np.random.seed(11)
+    rw = ResourceWorld(sw, [WorldConfig(**wk)], device_rng=True)
+    rw.create_resources()
+    with pytest.raises(Exception, match=TIMEOUT_MESSAGE):
+        rw.check()
 
 
 def test_world_rejects_bad_patch_count(gpu):
