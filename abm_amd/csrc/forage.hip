@@ -28,6 +28,7 @@
 #include "../../include/abm_amd.h"
 #include "forage_batch.hpp"
 #include "vf_exact.hpp"
+#include "vf_f32_common.hpp"
 
 namespace abm {
 int set_error(int code, const char* fmt, ...);
@@ -300,13 +301,81 @@ __device__ inline int cue_class(const FGSmem& S, const FGRep& P, int j, int f, d
 // exclusion every obstacle in range (agent.py:415-419).
 __device__ inline bool needs_record(int c, bool excl) { return c == kSocial || (excl && c != kNone); }
 
-// Record of obstacle j (class c) for focal geometry f.
+// Per-replica float32 constants of the guarded record.
+struct FastRec {
+    float inv_step;   // bins of linspace(-pi, pi, R) per radian
+    float hs;         // R / (2 pi): proj_size / 2 per radian of atan(r / d)
+    float g_bins;     // bin-tie guard in bins (angle guard + float32 rounding of the coordinate)
+    float fov0, fov1;
+};
+
+__device__ inline FastRec fast_rec_const(int R, double step, double fov0, double fov1) {
+    FastRec q;
+    q.inv_step = (float)(1.0 / step);
+    q.hs = (float)((double)R / kTwoPi);
+    q.g_bins = (float)(f32::kAngleGuardF / step + 1.5e-7 * R);
+    q.fov0 = (float)fov0;
+    q.fov1 = (float)fov1;
+    return q;
+}
+
+// Record of obstacle j (class c) for focal geometry f.  The bin, the
+// interval and the FOV test come from a float32 decision when it is outside
+// every guard band, else from the exact float64 record (agent.py:501-556):
+//  * the centre difference (focal radius for both centres) in float64,
+//    rounded once, and the focal's float64 unit heading: with atan2_f32 the
+//    signed angle a from heading to object is within 1e-6 rad of the exact
+//    one, and the reference's acos-based angle within 5e-8 rad of that;
+//  * the reference's closed angle is -a for 0 < |a| < pi (a' = a mod 2 pi,
+//    ca = -a' or 2 pi - a'); a = 0 gives 2 pi and |a| = pi gives pi: 4e-6 rad
+//    bands around 0 and pi, and around the strict FOV ends;
+//  * t = nearest bin of ca (bin-tie band as in the flocking kernels);
+//  * start / end = int(t -+ proj_size / 2), truncated toward zero: with half
+//    = proj_size / 2 outside a 4e-6 relative band around an integer, start =
+//    t - floor(half) - 1 when that stays positive, else t - floor(half),
+//    and end = t + floor(half).
+// The float64 distance, which only visual exclusion ranks by, is exact.
 __device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Focal& f, int R, double step,
-                                double fov0, double fov1) {
+                                double fov0, double fov1, const FastRec& FR, bool need_dist) {
+#pragma clang fp contract(off)
     Rec rc;
     rc.valid = 0; rc.social = 0; rc.order = 0; rc.start = rc.end = rc.sx = rc.ex = 0;
     rc.dist = 0.0; rc.vis = 0.0;
-    const PairFG p = pair_forage_exact(f, S.x[j], S.y[j], R, step, fov0, fov1);
+    const double ox = S.x[j], oy = S.y[j];
+    if (ox == f.px && oy == f.py) return rc;                 // skipped (agent.py:502)
+    const double v2x = (ox + f.radius) - f.cx, v2y = (oy + f.radius) - f.cy;
+    {
+        const float dx = (float)v2x, dy = (float)v2y;
+        const float u1x = (float)f.u1x, u1y = (float)f.u1y;
+        const float d2 = fmaf(dy, dy, dx * dx);
+        const float a = f32::atan2_f32(fmaf(u1x, dy, -u1y * dx), fmaf(u1x, dx, u1y * dy));
+        const float aa = fabsf(a);
+        const float ca = -a;
+        const float xb = (f32::kPiF - a) * FR.inv_step;     // bin coordinate of ca
+        const float tr = rintf(xb);
+        const float dfr = xb - tr;
+        const float half = f32::atan_pos_f32((float)f.radius, __builtin_amdgcn_sqrtf(d2)) * FR.hs;
+        const float hfl = floorf(half);
+        const float hfrac = half - hfl;
+        const float hg = 4e-6f * half + 1e-6f;
+        const float g = f32::kAngleGuardF;
+        const bool ok = (d2 > 1e-30f) & (aa > g) & (aa < f32::kPiF - g) &
+                        (fabsf(dfr) <= 0.5f - FR.g_bins) & (hfrac >= hg) & (hfrac <= 1.0f - hg) &
+                        (fabsf(ca - FR.fov0) >= g) & (fabsf(ca - FR.fov1) >= g);
+        if (ok) {
+            if (FR.fov0 < ca && ca < FR.fov1) {
+                const int t = (int)tr, h = (int)hfl;
+                rc.valid = 1;
+                rc.social = c == kSocial;
+                rc.order = c * N + j;
+                rc.start = rc.sx = t - h >= 1 ? t - h - 1 : t - h;
+                rc.end = rc.ex = t + h;
+                if (need_dist) rc.dist = norm2(v2x, v2y);
+            }
+            return rc;
+        }
+    }
+    const PairFG p = pair_forage_exact(f, ox, oy, R, step, fov0, fov1);
     if (!p.skipped && p.visible) {
         rc.valid = 1;
         rc.social = c == kSocial;
@@ -631,6 +700,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     int draws = 0;   // meaningful in thread 0
     const int nl = R / 2;
     const bool excl = P.visual_exclusion != 0;
+    const FastRec FR = fast_rec_const(R, P.step, P.fov0, P.fov1);
     int* const lst[2] = {S.lst, S.lst + maxN};
     int* const clsb[2] = {S.cls, S.cls2};
     int* const lst_n = S.cnt + 4;
@@ -658,7 +728,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     __syncthreads();
     for (int q = tid; q < lst_n[0]; q += nthr) {
         const int j = lst[0][q];
-        S.rec[j] = record_of(S, j, clsb[0][j], N, S.focal[0], R, P.step, P.fov0, P.fov1);
+        S.rec[j] = record_of(S, j, clsb[0][j], N, S.focal[0], R, P.step, P.fov0, P.fov1, FR, excl);
     }
     __syncthreads();
     raster_list(S, R, excl, lst[0], lst_n[0], -1);
@@ -734,7 +804,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             const int c = cue_class(S, P, k, f1, cfx, cfy, S.pid[f1]);
             int extra = -1;
             if (needs_record(c, excl)) {
-                S.rec[k] = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1);
+                S.rec[k] = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1, FR, excl);
                 extra = k;
             }
             S.cnt[6] = extra;
@@ -746,7 +816,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             const Focal f = S.focal[par];
             for (int q = tid - rbase; q < lst_n[par]; q += 64) {
                 const int j = lst[par][q];
-                S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1);
+                S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
             }
         }
         // ---- waves 2, 3: agent k+2's classes but agent k's and k+1's

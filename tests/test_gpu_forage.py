@@ -163,3 +163,61 @@ def test_batched_replicas_with_own_streams(gpu):
             assert oracle_rngs[b].get_state()[2] == rngs[b].get_state()[2]
             assert np.array_equal(oracle_rngs[b].get_state()[1], rngs[b].get_state()[1])
             compare(sw, b, ags)
+
+
+@pytest.mark.parametrize("fov_ratio", [1.0, 0.5])
+def test_guard_bands_of_the_float32_records(gpu, fov_ratio):
+    # every agent exploits its own patch, so each is a cue for every other;
+    # around agent 0 the obstacles sit 1e-9 inside and 1e-5 outside each guard
+    # band of the float32 record (bin ties, integer half-widths, dead ahead and
+    # behind, the FOV ends): inside, the exact record decides; outside, the
+    # float32 one.  Fields, and the one step, must match the oracle.
+    from abm_amd.forage import ForageConfig, ForageSwarm
+    R, r, W = 1200, 10, 500
+    fov = (-fov_ratio * PI, fov_ratio * PI)
+    step = 2 * PI / (R - 1)
+    th0 = 0.3
+    px = py = 250.0
+    u1 = np.array([np.cos(th0), -np.sin(th0)])
+
+    def at(a, d):
+        # object whose centre lies at signed angle a (cross > 0 for a > 0) from the heading
+        u2 = np.array([u1[0] * np.cos(a) - u1[1] * np.sin(a), u1[0] * np.sin(a) + u1[1] * np.cos(a)])
+        return px + d * u2[0], py + d * u2[1]
+
+    pos = []
+    for t in (100, 333, 600, 850, 1100):          # bin ties: ca = -pi + (t + 1/2) step
+        for e in (1e-9, -1e-9, 1e-5, -1e-5):
+            pos.append(at(-(-PI + (t + 0.5) * step) + e, 80.0 + 7 * len(pos) % 90))
+    for k in (1, 2, 3, 5):                          # half-width = k exactly at d_k
+        dk = r / np.tan(2 * PI * k / R)
+        for e in (1e-9, -1e-9, 1e-5, -1e-5):
+            pos.append(at(0.4 + 0.1 * k, dk * (1 + e)))
+    for a in (1e-9, -1e-9, 1e-5, -1e-5, PI - 1e-9, -(PI - 1e-9), PI - 1e-5, -(PI - 1e-5)):
+        pos.append(at(a, 60.0))                     # ahead and behind
+    for e in (1e-9, -1e-9, 1e-5, -1e-5):            # the FOV ends: ca = fov0, fov1
+        pos.append(at(-(fov[1] - e), 120.0))
+        pos.append(at(-(fov[0] + e), 140.0))
+    p = fo.ForageParams(Eps_w=3.0, Eps_u=1.0, F_N=0.5, F_R=0.5)
+    ags = [fo.ForageAgent(0, r, (px, py), th0, R, fov, W, W, 30, 2000.0, False, True, p)]
+    for j, (x, y) in enumerate(pos):
+        ags.append(fo.ForageAgent(j + 1, r, (x, y), 0.1 * j, R, fov, W, W, 30, 2000.0, False, True, p))
+    for i, a in enumerate(ags):
+        a.w, a.u, a.env_status, a.exploited_patch_id = 0.9, 0.9, 1, i
+        a.set_mode("exploit")
+    N = len(ags)
+    kw = dict(width=W, height=W, window_pad=30, fov=fov, vision_range=2000.0, visual_exclusion=False,
+              patchwise_exclusion=True, T_w=p.T_w, Eps_w=p.Eps_w, g_w=p.g_w, B_w=p.B_w, w_max=p.w_max,
+              T_u=p.T_u, Eps_u=p.Eps_u, g_u=p.g_u, B_u=p.B_u, u_max=p.u_max, S_wu=p.S_wu,
+              S_uw=p.S_uw, Tau=p.Tau, F_N=p.F_N, F_R=p.F_R, exp_vel_max=p.exp_vel_max,
+              exp_theta_min=p.exp_theta_min, exp_theta_max=p.exp_theta_max,
+              reloc_theta_max=p.reloc_theta_max, exp_stop_ratio=p.exp_stop_ratio)
+    sw = ForageSwarm([ForageConfig(n_agents=N, n_rays=R, **kw)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=r,
+                     rngs=[np.random.RandomState(3)])
+    load_swarm(sw, ags, sw.replica_slice(0))
+    np.random.seed(3)
+    fo.group_update(ags)
+    sw.step(fields=True)
+    assert ags[0].soc_v_field.any()
+    compare(sw, 0, ags)
