@@ -521,6 +521,79 @@ __device__ void raster_list(const FGSmem& S, int R, bool excl, const int* lst, i
     __syncthreads();
 }
 
+// Bits [lo, hi) of a field that fall in word w.
+__device__ inline uint32_t word_slice_mask(int w, int lo, int hi) {
+    return hi > lo ? word_range_mask(w, lo, hi - 1) : 0u;
+}
+
+// Sum over the 64 lanes of wave 0 with DPP moves (no LDS round trips):
+// prefix sums inside each row of 16 lanes (row_shr 1, 2, 4, 8), then row
+// 15's and row 31's lane carried into the rows above (row_bcast 15, 31), so
+// lane 63 holds the total.  Lanes a move has no source for add 0.
+__device__ inline int wave_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Raster and counts of the listed social records without visual exclusion
+// (agent.py:569-590, then the counts of count_field below) in wave 0 alone:
+// each lane builds raw-field words lane, lane + 64, ... in registers from
+// the same pieces paint_bits ORs, so there is no LDS field, no atomic and
+// no barrier.  cnt[0] = rays set on the kept range, cnt[1] = those in its
+// left half.
+__device__ void raster_count_wave(const FGSmem& S, int R, int W, const int* lst, int n0, int extra, int ta,
+                                  int tb, int la, int* cnt) {
+    const int lane = threadIdx.x & 63;
+    const int n = n0 + (extra >= 0 ? 1 : 0);
+    int tot = 0, left = 0;
+    for (int w = lane; w - lane < W; w += 64) {   // uniform trip count over the wave
+        uint32_t bits = 0u;
+        // 64 records at a time: lane q loads record q, then each live one's
+        // interval goes to every lane by readlane
+        for (int base = 0; base < n; base += 64) {
+            const int q = base + lane;
+            int sx = 0, ex = 0;
+            bool live = false;
+            if (q < n) {
+                const Rec& rc = S.rec[q < n0 ? lst[q] : extra];
+                live = rc.valid && rc.social;
+                sx = rc.sx;
+                ex = rc.ex;
+            }
+            uint64_t m = __ballot(live);
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                int s = __builtin_amdgcn_readlane(sx, l), e = __builtin_amdgcn_readlane(ex, l), lo, hi;
+                if (s < 0) {
+                    py_slice(R + s, R, R, &lo, &hi);
+                    bits |= word_slice_mask(w, lo, hi);
+                    s = 0;
+                }
+                if (e >= R) {
+                    py_slice(0, e - R, R, &lo, &hi);
+                    bits |= word_slice_mask(w, lo, hi);
+                    e = R - 1;
+                }
+                py_slice(s, e, R, &lo, &hi);
+                bits |= word_slice_mask(w, lo, hi);
+            }
+        }
+        if (w < W) {
+            tot += __popc(bits & word_range_mask(w, ta, tb));
+            left += __popc(bits & word_range_mask(w, la, tb));
+        }
+    }
+    tot = wave_sum(tot);
+    left = wave_sum(left);
+    if (lane == 0) { cnt[0] = tot; cnt[1] = left; }
+}
+
 // The returned field of agent k: counts for mean() and the left/right
 // halves into cnt; soc_v_field[r] = V[R-1-r] on the rays the FOV mask keeps.
 __device__ void count_field(const FGSmem& S, const FGRep& P, int R, int* cnt, uint32_t* bits_out, int k) {
@@ -639,9 +712,16 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // start at once and the small replicas fill in behind them
     const int rb = order[blockIdx.x];
     if (status && status[rb] != kWorldReady) return;   // uniform over the block
-    const FGRep P = reps[rb];
-    const int N = P.n_agents, R = P.n_rays, a0 = P.agent_offset;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // the replica's parameters in LDS: read where used, rather than held in
+    // scalar registers over the whole agent loop (which spills them)
+    __shared__ FGRep sP;
+    static_assert(sizeof(FGRep) % 8 == 0 && sizeof(FGRep) / 8 <= 64, "FGRep copy");
+    if (tid < (int)(sizeof(FGRep) / 8))
+        reinterpret_cast<uint64_t*>(&sP)[tid] = reinterpret_cast<const uint64_t*>(reps + rb)[tid];
+    __syncthreads();
+    const FGRep& P = sP;
+    const int N = P.n_agents, R = P.n_rays, a0 = P.agent_offset;
     FGSmem S = fg_carve(smem, maxN, maxR);
     for (int i = tid; i < N; i += nthr) {
         const int g = a0 + i;
@@ -701,6 +781,9 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     const int nl = R / 2;
     const bool excl = P.visual_exclusion != 0;
     const FastRec FR = fast_rec_const(R, P.step, P.fov0, P.fov1);
+    // without exclusion and field output, wave 0 rasters and counts alone
+    const bool wave_raster = !excl && !bits_out;
+    const int ta = R - 1 - P.keep_hi, tb = R - 1 - P.keep_lo, la = max(R - nl, ta);
     int* const lst[2] = {S.lst, S.lst + maxN};
     int* const clsb[2] = {S.cls, S.cls2};
     int* const lst_n = S.cnt + 4;
@@ -708,12 +791,13 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // agents moved before it only through their records.  So while thread 0
     // moves agent k and then adds agent k's class (and record) for agent
     // k+1, wave 1 builds agent k+1's records of every other obstacle and
-    // waves 2 and 3 classify agent k+2's obstacles but k and k+1.  After one
-    // barrier come exclusion, raster and counts of agent k+1, during which
-    // wave 2 adds agent k's class for agent k+2 and wave 3 builds agent
-    // k+2's focal geometry (agents move only in their own turn, so it is the
-    // start-of-step pose).  Classes, lists, focal geometry and counts
-    // alternate between two buffers by parity of the focal agent.
+    // then agent k+2's focal geometry (agents move only in their own turn,
+    // so it is the start-of-step pose), and waves 2 and 3 classify agent
+    // k+2's obstacles but k and k+1.  After one barrier come exclusion,
+    // raster and counts of agent k+1 (wave 0 alone when there is no
+    // exclusion), during which wave 2 adds agent k's class for agent k+2.
+    // Classes, lists, focal geometry and counts alternate between two
+    // buffers by parity of the focal agent.
     const int rbase = 64, cbase = 128, cstride = nthr - 128;   // blockDim 256
     if (tid == 0) {
         for (int q = 0; q < 8; ++q) S.cnt[q] = 0;
@@ -818,6 +902,9 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                 const int j = lst[par][q];
                 S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
             }
+            // agent k+2's focal geometry (its start-of-step pose: agents move
+            // only in their own turn); that buffer was last read for agent k
+            if (tid == rbase && f2 < N) S.focal[par2] = focal_of(S, f2);
         }
         // ---- waves 2, 3: agent k+2's classes but agent k's and k+1's
         if (f2 < N && tid >= cbase)
@@ -825,8 +912,13 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         __syncthreads();
         if (f1 >= N) break;
         // ---- exclusion, raster and counts of agent k+1; meanwhile wave 2
-        // adds agent k's class for agent k+2 and wave 3 its focal geometry
-        raster_list(S, R, excl, lst[par], lst_n[par], S.cnt[6]);
+        // adds agent k's class for agent k+2
+        if (wave_raster) {
+            if (tid < 64)
+                raster_count_wave(S, R, P.words, lst[par], lst_n[par], S.cnt[6], ta, tb, la, S.cnt + 2 * par);
+        } else {
+            raster_list(S, R, excl, lst[par], lst_n[par], S.cnt[6]);
+        }
         if (tid == 0) lst_n[par] = 0;   // rebuilt for agent k+3
         if (tid == cbase && f2 < N) {   // agent k's class for agent k+2; its record comes with the others
             const double gx = S.x[f2] + S.rad[f2], gy = S.y[f2] + S.rad[f2];
@@ -837,8 +929,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                 lst_n[par2] += 1;
             }
         }
-        if (tid == nthr - 64 && f2 < N) S.focal[par2] = focal_of(S, f2);
-        count_field(S, P, R, S.cnt + 2 * par, bits_out, f1);
+        if (!wave_raster) count_field(S, P, R, S.cnt + 2 * par, bits_out, f1);
         __syncthreads();
     }
     for (int i = tid; i < N; i += nthr) {
@@ -1317,9 +1408,9 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                                  b->device), "hipDeviceGetAttribute"))
         return rc;
-    if (lds > (size_t)dev_max)
+    if (lds + sizeof(FGRep) > (size_t)dev_max)   // the parameters sit in static LDS
         return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_step_f64: %d agents x %d rays need %zu B of LDS",
-                         b->max_agents, b->max_rays, lds);
+                         b->max_agents, b->max_rays, lds + sizeof(FGRep));
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),

@@ -155,7 +155,10 @@ def test_world_matches_oracle(gpu, regen_words, device_rng):
     kills = [0] * len(oracle)
     first = [[(p.id, p.position[0], p.position[1]) for p in w.patches] for w in oracle]
     for step in range(80):
-        rw.step(fields=True)
+        # odd steps without field output: wave 0 rasters and counts alone
+        # where there is no visual exclusion
+        with_fields = step % 2 == 0
+        rw.step(fields=with_fields)
         rw.sync_rng()
         for b, world in enumerate(oracle):
             before = [(p.id, p.position[0], p.position[1]) for p in world.patches]
@@ -166,8 +169,9 @@ def test_world_matches_oracle(gpu, regen_words, device_rng):
             kills[b] += len(set(before) - set(after))
             assert same_stream(rngs[b], orngs[b]), (step, b)
             compare(sw, rw, world, b, (step, b))
-            assert np.array_equal(sw.fields_numpy(b),
-                                  (fo.agents_arrays(world.agents)["soc_v_field"] != 0).astype(np.uint8))
+            if with_fields:
+                assert np.array_equal(sw.fields_numpy(b),
+                                      (fo.agents_arrays(world.agents)["soc_v_field"] != 0).astype(np.uint8))
             load_agents(sw, rw, world.agents, b)
     # depletion, regeneration (ids kept) and removal all happened
     assert kills[0] > 0 and kills[2] > 0 and kills[3] > 0, kills
