@@ -540,58 +540,80 @@ __device__ inline int wave_sum(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
-// Raster and counts of the listed social records without visual exclusion
-// (agent.py:569-590, then the counts of count_field below) in wave 0 alone:
-// each lane builds raw-field words lane, lane + 64, ... in registers from
-// the same pieces paint_bits ORs, so there is no LDS field, no atomic and
-// no barrier.  cnt[0] = rays set on the kept range, cnt[1] = those in its
-// left half.
-__device__ void raster_count_wave(const FGSmem& S, int R, int W, const int* lst, int n0, int extra, int ta,
-                                  int tb, int la, int* cnt) {
+__device__ inline void fg_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Raw-field word w of one social interval [s, e] of a record: the pieces
+// paint_bits ORs (agent.py:577-590).
+__device__ inline uint32_t interval_word(int w, int s, int e, int R) {
+    uint32_t bits = 0u;
+    int lo, hi;
+    if (s < 0) {
+        py_slice(R + s, R, R, &lo, &hi);
+        bits |= word_slice_mask(w, lo, hi);
+        s = 0;
+    }
+    if (e >= R) {
+        py_slice(0, e - R, R, &lo, &hi);
+        bits |= word_slice_mask(w, lo, hi);
+        e = R - 1;
+    }
+    py_slice(s, e, R, &lo, &hi);
+    return bits | word_slice_mask(w, lo, hi);
+}
+
+// Raster without visual exclusion, by one wave: raw-field word w of the
+// live social records lst[0..n).  Lane l loads records l, l + 64, ...; each
+// live interval reaches every lane by readlane, so there is no atomic.
+__device__ inline uint32_t list_word(const FGSmem& S, int R, const int* lst, int n, int w) {
     const int lane = threadIdx.x & 63;
-    const int n = n0 + (extra >= 0 ? 1 : 0);
-    int tot = 0, left = 0;
-    for (int w = lane; w - lane < W; w += 64) {   // uniform trip count over the wave
-        uint32_t bits = 0u;
-        // 64 records at a time: lane q loads record q, then each live one's
-        // interval goes to every lane by readlane
-        for (int base = 0; base < n; base += 64) {
-            const int q = base + lane;
-            int sx = 0, ex = 0;
-            bool live = false;
-            if (q < n) {
-                const Rec& rc = S.rec[q < n0 ? lst[q] : extra];
-                live = rc.valid && rc.social;
-                sx = rc.sx;
-                ex = rc.ex;
-            }
-            uint64_t m = __ballot(live);
-            while (m) {
-                const int l = __builtin_ctzll(m);
-                m &= m - 1;
-                int s = __builtin_amdgcn_readlane(sx, l), e = __builtin_amdgcn_readlane(ex, l), lo, hi;
-                if (s < 0) {
-                    py_slice(R + s, R, R, &lo, &hi);
-                    bits |= word_slice_mask(w, lo, hi);
-                    s = 0;
-                }
-                if (e >= R) {
-                    py_slice(0, e - R, R, &lo, &hi);
-                    bits |= word_slice_mask(w, lo, hi);
-                    e = R - 1;
-                }
-                py_slice(s, e, R, &lo, &hi);
-                bits |= word_slice_mask(w, lo, hi);
-            }
+    uint32_t bits = 0u;
+    for (int base = 0; base < n; base += 64) {
+        const int q = base + lane;
+        int sx = 0, ex = 0;
+        bool live = false;
+        if (q < n) {
+            const Rec& rc = S.rec[lst[q]];
+            live = rc.valid && rc.social;
+            sx = rc.sx;
+            ex = rc.ex;
         }
-        if (w < W) {
-            tot += __popc(bits & word_range_mask(w, ta, tb));
-            left += __popc(bits & word_range_mask(w, la, tb));
+        uint64_t m = __ballot(live);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            bits |= interval_word(w, __builtin_amdgcn_readlane(sx, l), __builtin_amdgcn_readlane(ex, l), R);
         }
     }
-    tot = wave_sum(tot);
-    left = wave_sum(left);
-    if (lane == 0) { cnt[0] = tot; cnt[1] = left; }
+    return bits;
+}
+
+// Counts of the field without visual exclusion, by wave 0: the words the
+// listed records made (S.V), with the interval [s, e] of the record kept
+// outside the list ORed in when live (wave-uniform): the rays set on the
+// kept raw range [ta, tb] and those in its left half [la, tb], to every lane.
+__device__ void count_wave(const FGSmem& S, int R, int W, bool live, int s, int e, int ta, int tb, int la,
+                           int* n_set, int* n_left) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t* Vb = reinterpret_cast<const uint32_t*>(S.V);
+    int tot = 0, left = 0;
+    for (int w = lane; w < W; w += 64) {
+        uint32_t bits = Vb[w];
+        if (live) bits |= interval_word(w, s, e, R);
+        tot += __popc(bits & word_range_mask(w, ta, tb));
+        left += __popc(bits & word_range_mask(w, la, tb));
+    }
+    if (R < 65536) {   // one sum of both counts, each below 2^16
+        const int both = wave_sum(tot << 16 | left);
+        *n_set = both >> 16;
+        *n_left = both & 0xffff;
+    } else {
+        *n_set = wave_sum(tot);
+        *n_left = wave_sum(left);
+    }
 }
 
 // The returned field of agent k: counts for mean() and the left/right
@@ -699,7 +721,7 @@ __device__ inline double words_to_double(uint32_t w0, uint32_t w1) {
 // 32-bit outputs (words, two per draw, starting after the words the resource
 // loop consumed: u_used[b] on entry).  With status, only replicas marked
 // kWorldReady are updated, and they are marked ABM_WORLD_DONE.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
                    const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
@@ -778,6 +800,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         }
     }
     int draws = 0;   // meaningful in thread 0
+    int xlive = 0, xs = 0, xe = 0;   // thread 0: agent k's social interval for agent k+1
     const int nl = R / 2;
     const bool excl = P.visual_exclusion != 0;
     const FastRec FR = fast_rec_const(R, P.step, P.fov0, P.fov1);
@@ -790,15 +813,17 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // Pipeline over the agents in id order.  Agent m's field depends on the
     // agents moved before it only through their records.  So while thread 0
     // moves agent k and then adds agent k's class (and record) for agent
-    // k+1, wave 1 builds agent k+1's records of every other obstacle and
-    // then agent k+2's focal geometry (agents move only in their own turn,
-    // so it is the start-of-step pose), and waves 2 and 3 classify agent
-    // k+2's obstacles but k and k+1.  After one barrier come exclusion,
-    // raster and counts of agent k+1 (wave 0 alone when there is no
-    // exclusion), during which wave 2 adds agent k's class for agent k+2.
+    // k+1, wave 1 builds agent k+1's records of every other obstacle
+    // (without exclusion, also their raster), and waves 2 to 7 classify
+    // agent k+2's obstacles but k and k+1, the last of them then building
+    // agent k+2's focal geometry (agents move only in their own turn, so it
+    // is the start-of-step pose).  After one barrier come exclusion, raster and
+    // counts of agent k+1 (without exclusion: agent k's interval ORed in and
+    // the counts, by wave 0 alone), during which wave 2 adds agent k's class
+    // for agent k+2.
     // Classes, lists, focal geometry and counts alternate between two
     // buffers by parity of the focal agent.
-    const int rbase = 64, cbase = 128, cstride = nthr - 128;   // blockDim 256
+    const int rbase = 64, cbase = 128, cstride = nthr - 128;   // blockDim 256 or 512
     if (tid == 0) {
         for (int q = 0; q < 8; ++q) S.cnt[q] = 0;
     }
@@ -820,12 +845,16 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     count_field(S, P, R, S.cnt, bits_out, 0);
     __syncthreads();
 
+    // wave 0 carries the chain of moves: first issue on its SIMD.  It holds
+    // the focal agent's counts in registers (from LDS after count_field).
+    if (tid < 64) __builtin_amdgcn_s_setprio(3);
+    int n_set = S.cnt[0], n_left = S.cnt[1];
     for (int k = 0; k < N; ++k) {
         const int f1 = k + 1, par = f1 & 1, f2 = k + 2, par2 = par ^ 1;
-        const int* cnt = S.cnt + 2 * (k & 1);
+        if (!wave_raster && k > 0) { n_set = S.cnt[2 * (k & 1)]; n_left = S.cnt[2 * (k & 1) + 1]; }
         // ---- decision and motion (agent.py:168-283)
         if (tid == 0) {
-            const double count = (double)cnt[0], cleft = (double)cnt[1];
+            const double count = (double)n_set, cleft = (double)n_left;
             double w = S.w[k], u = S.u[k], vel0 = S.v[k];
             const double collected = S.cr[k] - S.crb[k];
             const double ipriv = P.F_N * S.nov[k] + P.F_R * collected;
@@ -887,35 +916,52 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             const double cfx = S.x[f1] + S.rad[f1], cfy = S.y[f1] + S.rad[f1];
             const int c = cue_class(S, P, k, f1, cfx, cfy, S.pid[f1]);
             int extra = -1;
+            xlive = 0;
             if (needs_record(c, excl)) {
-                S.rec[k] = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1, FR, excl);
+                const Rec rk = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1, FR, excl);
+                S.rec[k] = rk;
                 extra = k;
+                xlive = rk.valid && rk.social;
+                xs = rk.sx;
+                xe = rk.ex;
             }
             S.cnt[6] = extra;
             S.cnt[2 * par] = 0; S.cnt[2 * par + 1] = 0;   // agent k+1's counts (last read by k-1)
         }
         // ---- wave 1: agent k+1's records but agent k's
         if (f1 < N && tid >= rbase && tid < cbase) {
-            for (int w = tid - rbase; w < P.words; w += 64) reinterpret_cast<uint32_t*>(S.V)[w] = 0u;
+            uint32_t* Vb = reinterpret_cast<uint32_t*>(S.V);
+            if (!wave_raster)
+                for (int w = tid - rbase; w < P.words; w += 64) Vb[w] = 0u;
             const Focal f = S.focal[par];
             for (int q = tid - rbase; q < lst_n[par]; q += 64) {
                 const int j = lst[par][q];
                 S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
             }
-            // agent k+2's focal geometry (its start-of-step pose: agents move
-            // only in their own turn); that buffer was last read for agent k
-            if (tid == rbase && f2 < N) S.focal[par2] = focal_of(S, f2);
+            if (wave_raster) {   // the listed records' raster; agent k's comes in phase 2
+                fg_wave_sync();
+                const int wl = tid - rbase;
+                for (int w = wl; w - wl < P.words; w += 64) {
+                    const uint32_t bits = list_word(S, R, lst[par], lst_n[par], w);
+                    if (w < P.words) Vb[w] = bits;
+                }
+            }
         }
-        // ---- waves 2, 3: agent k+2's classes but agent k's and k+1's
+        // ---- waves 2 to 7: agent k+2's classes but agent k's and k+1's
         if (f2 < N && tid >= cbase)
             classify(S, P, N, f2, k, f1, excl, clsb[par2], lst[par2], &lst_n[par2], tid - cbase, cstride);
+        // ---- the last wave, then: agent k+2's focal geometry (its
+        // start-of-step pose: agents move only in their own turn); that
+        // buffer was last read for agent k
+        if (tid == nthr - 64 && f2 < N) S.focal[par2] = focal_of(S, f2);
         __syncthreads();
         if (f1 >= N) break;
         // ---- exclusion, raster and counts of agent k+1; meanwhile wave 2
         // adds agent k's class for agent k+2
-        if (wave_raster) {
+        if (wave_raster) {   // agent k's interval from thread 0's registers
             if (tid < 64)
-                raster_count_wave(S, R, P.words, lst[par], lst_n[par], S.cnt[6], ta, tb, la, S.cnt + 2 * par);
+                count_wave(S, R, P.words, __builtin_amdgcn_readlane(xlive, 0) != 0, __builtin_amdgcn_readlane(xs, 0),
+                           __builtin_amdgcn_readlane(xe, 0), ta, tb, la, &n_set, &n_left);
         } else {
             raster_list(S, R, excl, lst[par], lst_n[par], S.cnt[6]);
         }
@@ -1400,9 +1446,11 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
                        const uint32_t* words, const int64_t* offset, const int32_t* count,
                        int32_t* used, int32_t* status, uint32_t* field_bits, void* stream,
                        abm_mt19937_state* mt = nullptr) {
-    // four waves: the step kernel overlaps one agent's move (wave 0) with
-    // the next agent's records (wave 1) and classes (waves 2, 3)
-    const int nthr = 256;
+    // the step kernel overlaps one agent's move (wave 0) with the next
+    // agent's records (wave 1) and the classes of the one after (the other
+    // waves): eight waves when some replica has more agents than two waves
+    // classify in one pass, else four (two workgroups then share a CU)
+    const int nthr = b->max_agents > 128 ? 512 : 256;
     const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
     int dev_max = 0;
     if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,

@@ -167,6 +167,31 @@ def test_batched_replicas_with_own_streams(gpu):
             compare(sw, b, ags, fields=with_fields)
 
 
+@pytest.mark.parametrize("vis_ex", [False, True])
+def test_eight_wave_pipeline_matches_oracle(gpu, vis_ex):
+    # 240 agents: the step runs eight waves per replica (six classify), and
+    # over 64 exploiters see each other, so without exclusion the listed
+    # social records span two 64-record rounds of the wave raster
+    from abm_amd.forage import ForageConfig, ForageSwarm
+    N = 240
+    ags, kw = make_world(900 + vis_ex, N=N, R=256, W=800, vis_ex=vis_ex)
+    rng = np.random.RandomState(77)
+    sw = ForageSwarm([ForageConfig(n_agents=N, n_rays=256, **kw)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=10, rngs=[rng])
+    load_swarm(sw, ags, sw.replica_slice(0))
+    assert sum(a.mode == "exploit" for a in ags) > 64
+    orng = np.random.RandomState(77)
+    for step in range(3):
+        with_fields = step == 0
+        sw.step(fields=with_fields)
+        np.random.set_state(orng.get_state())
+        fo.group_update(ags)
+        orng.set_state(np.random.get_state())
+        assert orng.get_state()[2] == rng.get_state()[2]
+        assert np.array_equal(orng.get_state()[1], rng.get_state()[1])
+        compare(sw, 0, ags, fields=with_fields)
+
+
 @pytest.mark.parametrize("fov_ratio", [1.0, 0.5])
 def test_guard_bands_of_the_float32_records(gpu, fov_ratio):
     # every agent exploits its own patch, so each is a cue for every other;
