@@ -144,6 +144,8 @@ struct FGSmem {
     uint8_t* V;
     int* cnt;       // step kernel: [2p] total, [2p+1] left half by parity; [4 + p] list sizes
     double* lastd;  // keep_distance: distance of the last processed obstacle
+    double* spec;   // step kernel: [12 p ..] an agent's headings after a walk turn or none, with sin and
+                    // cos, by parity p; [24 + p] uniforms drawn before an agent of parity p
 };
 
 __device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
@@ -153,6 +155,7 @@ __device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
     double** dptrs[] = {&s.x, &s.y, &s.th, &s.v, &s.w, &s.u, &s.cr, &s.crb, &s.nov, &s.rad};
     for (double** d : dptrs) { *d = reinterpret_cast<double*>(p); p += sizeof(double) * maxN; }
     s.lastd = reinterpret_cast<double*>(p); p += sizeof(double) * 2;
+    s.spec = reinterpret_cast<double*>(p); p += sizeof(double) * 28;
     s.uni = reinterpret_cast<double*>(p); p += sizeof(double) * maxN;
     s.rec = reinterpret_cast<Rec*>(p); p += sizeof(Rec) * maxN;
     int** iptrs[] = {&s.mode, &s.over, &s.env, &s.pid, &s.cls};
@@ -169,7 +172,7 @@ __device__ inline FGSmem fg_carve(void* base, int maxN, int maxR) {
 static_assert(sizeof(Focal) <= 64, "two Focal slots of 64 bytes");
 
 size_t fg_lds_bytes(int maxN, int maxR) {
-    return 128 + sizeof(double) * ((size_t)11 * maxN + 2) + sizeof(Rec) * (size_t)maxN +
+    return 128 + sizeof(double) * ((size_t)11 * maxN + 30) + sizeof(Rec) * (size_t)maxN +
            sizeof(int) * ((size_t)9 * maxN + 8) + sizeof(uint32_t) * 624 + (size_t)maxR + 16;
 }
 
@@ -432,6 +435,29 @@ __device__ inline Focal focal_of(const FGSmem& S, int k) {
     f.u1x = v1x / n1;
     f.u1y = v1y / n1;
     return f;
+}
+
+// The heading agent f takes after a random-walk turn drawn from uniform
+// d + c (c = 0, 1) or after no turn (c = 2), with its sin and cos, into
+// buf[3c .. 3c + 2] (agent.py:187-200, 595-603: prove_orientation of the
+// turned heading, then cos and sin for the move); buf[9] = d.  The same
+// arithmetic as the move, computed by a spare wave before agent f's
+// decision: the walk's draw index is d or d + 1 (the agent before draws at
+// most one uniform), so the decision only picks a slot.
+__device__ inline void spec_heading(const FGSmem& S, const FGRep& P, int N, int f, int d, int c, double* buf) {
+#pragma clang fp contract(off)
+    double dth = 0.0;
+    if (c < 2) {
+        const double uu = d + c < N ? S.uni[d + c] : 0.0;
+        dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
+    }
+    const double th = prove_orientation(S.th[f] + dth);
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    buf[3 * c] = th;
+    buf[3 * c + 1] = sn;
+    buf[3 * c + 2] = cs;
+    if (c == 0) buf[9] = (double)d;
 }
 
 // Set bits [lo, hi) of a bit-packed field.
@@ -829,7 +855,11 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     }
     for (int w = tid; w < P.words; w += nthr) reinterpret_cast<uint32_t*>(S.V)[w] = 0u;
     __syncthreads();
-    // ---- agent 0's field; agent 1's classes but agent 0's
+    // ---- agent 0's field; agent 1's classes but agent 0's; agent 0's
+    // speculative headings (slot 0, draw index 0)
+    const int spec_tid = nthr - 128;   // first lanes of the second-to-last wave
+    if (tid >= spec_tid && tid < spec_tid + 3) spec_heading(S, P, N, 0, 0, tid - spec_tid, S.spec);
+    if (tid == 0) S.spec[24] = 0.0;    // uniforms drawn before agent 0
     if (tid == 0) S.focal[0] = focal_of(S, 0);
     if (tid == cbase && N > 1) S.focal[1] = focal_of(S, 1);
     classify(S, P, N, 0, -1, -1, excl, clsb[0], lst[0], &lst_n[0], tid, nthr);
@@ -871,11 +901,13 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             if (u < -P.u_max) u = -P.u_max;
             int over = S.over[k], mode = S.mode[k];
             double dvel = 0.0, dth = 0.0;
+            int slot = 2;   // speculative heading: 2 = no turn, 0 / 1 = walk, -1 = relocation
             const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
+            const double* spec = S.spec + 12 * (k & 1);
             auto walk = [&]() {
-                const double uu = S.uni[draws++];
+                slot = draws - (int)spec[9];   // the uniform drawn: S.uni[draws]
+                ++draws;
                 dvel = P.exp_vel_max;
-                dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
                 over = -1; mode = ABM_MODE_EXPLORE;
             };
             auto reloc = [&]() {
@@ -883,6 +915,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                 const double rm = (count - cleft) / (double)(R - nl);
                 dvel = P.exp_vel_max - vel0;
                 dth = (lm - rm) * P.reloc_theta_max;
+                slot = -1;
                 over = -1; mode = ABM_MODE_RELOCATE;
             };
             auto exploit = [&]() {
@@ -897,12 +930,18 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                 else { if (S.env[k] == 1) exploit(); else walk(); }
             }
             Pose p{S.x[k], S.y[k], S.th[k], vel0};
-            p.theta = prove_orientation(p.theta + dth);
+            double sn, cs;
+            if (slot < 0) {
+                p.theta = prove_orientation(p.theta + dth);
+                sincos(p.theta, &sn, &cs);
+            } else {   // prove_orientation(theta + turn) and its sin, cos from the spare wave
+                p.theta = spec[3 * slot];
+                sn = spec[3 * slot + 1];
+                cs = spec[3 * slot + 2];
+            }
             p.v = p.v + dvel;
             // prove_velocity (agent.py:612-620): only in explore mode
             if (get_mode(over, w, P.T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = P.exp_vel_max;
-            double sn, cs;
-            sincos(p.theta, &sn, &cs);
             p.x = p.x + p.v * cs;
             p.y = p.y - p.v * sn;
             p = reflect_walls(p, S.rad[k], P.pad, P.width, P.height);
@@ -910,6 +949,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             S.w[k] = w; S.u[k] = u; S.over[k] = over; S.mode[k] = mode;
             S.crb[k] = S.cr[k];
             S.nov[k] = ipriv;   // read only by agent k's decision: now its I_priv
+            S.spec[24 + (f1 & 1)] = (double)draws;   // uniforms drawn before agent k+1
         }
         // ---- thread 0, after the move: agent k's class and record for agent k+1
         if (tid == 0 && f1 < N) {
@@ -934,13 +974,36 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             if (!wave_raster)
                 for (int w = tid - rbase; w < P.words; w += 64) Vb[w] = 0u;
             const Focal f = S.focal[par];
-            for (int q = tid - rbase; q < lst_n[par]; q += 64) {
-                const int j = lst[par][q];
-                S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
+            const int n = lst_n[par], wl = tid - rbase;
+            if (wave_raster && n <= 64 && P.words <= 64) {
+                // one record and one field word per lane: the live intervals
+                // go from registers to every lane by readlane (the records
+                // are not needed after the raster)
+                int sx = 0, ex = 0;
+                bool live = false;
+                if (wl < n) {
+                    const int j = lst[par][wl];
+                    const Rec rc = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
+                    live = rc.valid && rc.social;
+                    sx = rc.sx;
+                    ex = rc.ex;
+                }
+                uint64_t m = __ballot(live);
+                uint32_t bits = 0u;
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    bits |= interval_word(wl, __builtin_amdgcn_readlane(sx, l), __builtin_amdgcn_readlane(ex, l), R);
+                }
+                if (wl < P.words) Vb[wl] = bits;
+            } else {
+                for (int q = wl; q < n; q += 64) {
+                    const int j = lst[par][q];
+                    S.rec[j] = record_of(S, j, clsb[par][j], N, f, R, P.step, P.fov0, P.fov1, FR, excl);
+                }
             }
-            if (wave_raster) {   // the listed records' raster; agent k's comes in phase 2
+            if (wave_raster && !(n <= 64 && P.words <= 64)) {   // the listed records' raster; agent k's comes in phase 2
                 fg_wave_sync();
-                const int wl = tid - rbase;
                 for (int w = wl; w - wl < P.words; w += 64) {
                     const uint32_t bits = list_word(S, R, lst[par], lst_n[par], w);
                     if (w < P.words) Vb[w] = bits;
@@ -950,6 +1013,10 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         // ---- waves 2 to 7: agent k+2's classes but agent k's and k+1's
         if (f2 < N && tid >= cbase)
             classify(S, P, N, f2, k, f1, excl, clsb[par2], lst[par2], &lst_n[par2], tid - cbase, cstride);
+        // ---- the second-to-last wave, then: agent k+1's speculative
+        // headings, from the uniforms drawn before agent k
+        if (f1 < N && tid >= spec_tid && tid < spec_tid + 3)
+            spec_heading(S, P, N, f1, (int)S.spec[24 + (k & 1)], tid - spec_tid, S.spec + 12 * par);
         // ---- the last wave, then: agent k+2's focal geometry (its
         // start-of-step pose: agents move only in their own turn); that
         // buffer was last read for agent k
