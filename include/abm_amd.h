@@ -325,7 +325,7 @@ int abm_forage_batch_get_offsets(const abm_forage_batch* batch, int32_t* agent_o
 /* One Agent.update for every agent, reference order, float64.  One
  * workgroup per replica (256 threads, or 512 when a replica has more than
  * 128 agents) holds the replica in LDS: about 172 bytes per agent plus one
- * byte per ray, so up to about 940 agents per replica at R = 1200 on gfx950
+ * byte per ray, so up to about 920 agents per replica at R = 1200 on gfx950
  * (160 KB); larger replicas get ABM_ERR_UNSUPPORTED. */
 /* Arguments:
  * uniforms: per replica, a device array of the next numpy legacy-MT19937
