@@ -840,10 +840,11 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // agents moved before it only through their records.  So while thread 0
     // moves agent k and then adds agent k's class (and record) for agent
     // k+1, wave 1 builds agent k+1's records of every other obstacle
-    // (without exclusion, also their raster), and waves 2 to 7 classify
-    // agent k+2's obstacles but k and k+1, the last of them then building
-    // agent k+2's focal geometry (agents move only in their own turn, so it
-    // is the start-of-step pose).  After one barrier come exclusion, raster and
+    // (without exclusion, also their raster), and the other waves classify
+    // agent k+2's obstacles but k and k+1; one of them then builds agent
+    // k+1's speculative headings and the next agent k+2's focal geometry
+    // (agents move only in their own turn, so it is the start-of-step
+    // pose).  After one barrier come exclusion, raster and
     // counts of agent k+1 (without exclusion: agent k's interval ORed in and
     // the counts, by wave 0 alone), during which wave 2 adds agent k's class
     // for agent k+2.
@@ -857,7 +858,10 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     __syncthreads();
     // ---- agent 0's field; agent 1's classes but agent 0's; agent 0's
     // speculative headings (slot 0, draw index 0)
-    const int spec_tid = nthr - 128;   // first lanes of the second-to-last wave
+    // two waves that classify once per focal agent also build the
+    // speculative headings and the focal geometry: waves 4 and 5 of eight
+    // (waves 2 and 3 classify twice for 512 agents), waves 2 and 3 of four
+    const int spec_tid = nthr == 512 ? 256 : 128, focal_tid = spec_tid + 64;
     if (tid >= spec_tid && tid < spec_tid + 3) spec_heading(S, P, N, 0, 0, tid - spec_tid, S.spec);
     if (tid == 0) S.spec[24] = 0.0;    // uniforms drawn before agent 0
     if (tid == 0) S.focal[0] = focal_of(S, 0);
@@ -953,10 +957,15 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         }
         // ---- thread 0, after the move: agent k's class and record for agent k+1
         if (tid == 0 && f1 < N) {
-            const double cfx = S.x[f1] + S.rad[f1], cfy = S.y[f1] + S.rad[f1];
-            const int c = cue_class(S, P, k, f1, cfx, cfy, S.pid[f1]);
             int extra = -1;
             xlive = 0;
+            // without exclusion only an exploiting agent k (overriding mode
+            // exploit) can be a cue for agent k+1: skip the distance otherwise
+            int c = kNone;
+            if (excl || S.over[k] == ABM_MODE_EXPLOIT) {
+                const double cfx = S.x[f1] + S.rad[f1], cfy = S.y[f1] + S.rad[f1];
+                c = cue_class(S, P, k, f1, cfx, cfy, S.pid[f1]);
+            }
             if (needs_record(c, excl)) {
                 const Rec rk = record_of(S, k, c, N, S.focal[par], R, P.step, P.fov0, P.fov1, FR, excl);
                 S.rec[k] = rk;
@@ -1013,14 +1022,14 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         // ---- waves 2 to 7: agent k+2's classes but agent k's and k+1's
         if (f2 < N && tid >= cbase)
             classify(S, P, N, f2, k, f1, excl, clsb[par2], lst[par2], &lst_n[par2], tid - cbase, cstride);
-        // ---- the second-to-last wave, then: agent k+1's speculative
-        // headings, from the uniforms drawn before agent k
+        // ---- one classifying wave, then: agent k+1's speculative headings,
+        // from the uniforms drawn before agent k
         if (f1 < N && tid >= spec_tid && tid < spec_tid + 3)
             spec_heading(S, P, N, f1, (int)S.spec[24 + (k & 1)], tid - spec_tid, S.spec + 12 * par);
-        // ---- the last wave, then: agent k+2's focal geometry (its
+        // ---- the next wave, then: agent k+2's focal geometry (its
         // start-of-step pose: agents move only in their own turn); that
         // buffer was last read for agent k
-        if (tid == nthr - 64 && f2 < N) S.focal[par2] = focal_of(S, f2);
+        if (tid == focal_tid && f2 < N) S.focal[par2] = focal_of(S, f2);
         __syncthreads();
         if (f1 >= N) break;
         // ---- exclusion, raster and counts of agent k+1; meanwhile wave 2
