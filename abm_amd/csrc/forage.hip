@@ -621,16 +621,24 @@ __device__ inline uint32_t list_word(const FGSmem& S, int R, const int* lst, int
 // listed records made (S.V), with the interval [s, e] of the record kept
 // outside the list ORed in when live (wave-uniform): the rays set on the
 // kept raw range [ta, tb] and those in its left half [la, tb], to every lane.
+// m_tot / m_left: the lane's word masks of the two ranges when W <= 64.
 __device__ void count_wave(const FGSmem& S, int R, int W, bool live, int s, int e, int ta, int tb, int la,
-                           int* n_set, int* n_left) {
+                           uint32_t m_tot, uint32_t m_left, int* n_set, int* n_left) {
     const int lane = threadIdx.x & 63;
     const uint32_t* Vb = reinterpret_cast<const uint32_t*>(S.V);
     int tot = 0, left = 0;
-    for (int w = lane; w < W; w += 64) {
-        uint32_t bits = Vb[w];
-        if (live) bits |= interval_word(w, s, e, R);
-        tot += __popc(bits & word_range_mask(w, ta, tb));
-        left += __popc(bits & word_range_mask(w, la, tb));
+    if (W <= 64) {   // one word per lane
+        uint32_t bits = lane < W ? Vb[lane] : 0u;
+        if (live) bits |= interval_word(lane, s, e, R);
+        tot = __popc(bits & m_tot);
+        left = __popc(bits & m_left);
+    } else {
+        for (int w = lane; w < W; w += 64) {
+            uint32_t bits = Vb[w];
+            if (live) bits |= interval_word(w, s, e, R);
+            tot += __popc(bits & word_range_mask(w, ta, tb));
+            left += __popc(bits & word_range_mask(w, la, tb));
+        }
     }
     if (R < 65536) {   // one sum of both counts, each below 2^16
         const int both = wave_sum(tot << 16 | left);
@@ -833,6 +841,10 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     // without exclusion and field output, wave 0 rasters and counts alone
     const bool wave_raster = !excl && !bits_out;
     const int ta = R - 1 - P.keep_hi, tb = R - 1 - P.keep_lo, la = max(R - nl, ta);
+    // wave 0's lane masks of the kept range and its left half (one word per lane)
+    const int wl0 = tid & 63;
+    const uint32_t m_tot = wl0 < P.words ? word_range_mask(wl0, ta, tb) : 0u;
+    const uint32_t m_left = wl0 < P.words ? word_range_mask(wl0, la, tb) : 0u;
     int* const lst[2] = {S.lst, S.lst + maxN};
     int* const clsb[2] = {S.cls, S.cls2};
     int* const lst_n = S.cnt + 4;
@@ -1037,14 +1049,19 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         if (wave_raster) {   // agent k's interval from thread 0's registers
             if (tid < 64)
                 count_wave(S, R, P.words, __builtin_amdgcn_readlane(xlive, 0) != 0, __builtin_amdgcn_readlane(xs, 0),
-                           __builtin_amdgcn_readlane(xe, 0), ta, tb, la, &n_set, &n_left);
+                           __builtin_amdgcn_readlane(xe, 0), ta, tb, la, m_tot, m_left, &n_set, &n_left);
         } else {
             raster_list(S, R, excl, lst[par], lst_n[par], S.cnt[6]);
         }
         if (tid == 0) lst_n[par] = 0;   // rebuilt for agent k+3
         if (tid == cbase && f2 < N) {   // agent k's class for agent k+2; its record comes with the others
-            const double gx = S.x[f2] + S.rad[f2], gy = S.y[f2] + S.rad[f2];
-            const int c2 = cue_class(S, P, k, f2, gx, gy, S.pid[f2]);
+            // without exclusion only a listed obstacle's class is read, and
+            // only an exploiting agent k can be listed: skip the distance otherwise
+            int c2 = kNone;
+            if (excl || S.over[k] == ABM_MODE_EXPLOIT) {
+                const double gx = S.x[f2] + S.rad[f2], gy = S.y[f2] + S.rad[f2];
+                c2 = cue_class(S, P, k, f2, gx, gy, S.pid[f2]);
+            }
             clsb[par2][k] = c2;
             if (needs_record(c2, excl)) {
                 lst[par2][lst_n[par2]] = k;
