@@ -404,7 +404,10 @@ __device__ void classify(const FGSmem& S, const FGRep& P, int N, int f, int skip
         const int j = base + lane;
         bool need = false;
         if (j < N && j != skip1 && j != skip2) {
-            const int c = cue_class(S, P, j, f, cfx, cfy, pid_f);
+            // without exclusion only a listed obstacle's class is read, and
+            // only an exploiting agent (overriding mode exploit) is listed
+            int c = kNone;
+            if (excl || S.over[j] == ABM_MODE_EXPLOIT) c = cue_class(S, P, j, f, cfx, cfy, pid_f);
             cls[j] = c;
             need = needs_record(c, excl);
         }
@@ -900,52 +903,64 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
         if (!wave_raster && k > 0) { n_set = S.cnt[2 * (k & 1)]; n_left = S.cnt[2 * (k & 1) + 1]; }
         // ---- decision and motion (agent.py:168-283)
         if (tid == 0) {
+            // every LDS operand first, in one batch: the replica's constants
+            // and agent k's state (the loads then wait once, not in turn)
+            const double T_w = P.T_w, Eps_w = P.Eps_w, g_w = P.g_w, B_w = P.B_w, w_max = P.w_max;
+            const double T_u = P.T_u, Eps_u = P.Eps_u, g_u = P.g_u, B_u = P.B_u, u_max = P.u_max;
+            const double S_wu = P.S_wu, S_uw = P.S_uw, F_N = P.F_N, F_R = P.F_R;
+            const double vmax = P.exp_vel_max, rtmax = P.reloc_theta_max, stop = P.exp_stop_ratio;
+            const double pad = P.pad, width = P.width, height = P.height;
+            double w = S.w[k], u = S.u[k];
+            const double vel0 = S.v[k], cr = S.cr[k], crb = S.crb[k], nov = S.nov[k];
+            const double x0 = S.x[k], y0 = S.y[k], th0 = S.th[k], rad = S.rad[k];
+            int over = S.over[k], mode = S.mode[k];
+            const int env = S.env[k];
+            const double* spec = S.spec + 12 * (k & 1);
+            const double spec_d = spec[9];
+            asm volatile("" ::: "memory");
             const double count = (double)n_set, cleft = (double)n_left;
-            double w = S.w[k], u = S.u[k], vel0 = S.v[k];
-            const double collected = S.cr[k] - S.crb[k];
-            const double ipriv = P.F_N * S.nov[k] + P.F_R * collected;
-            const double w_p = w > P.T_w ? w : 0.0;
-            const double u_p = u > P.T_u ? u : 0.0;
+            const double collected = cr - crb;
+            const double ipriv = F_N * nov + F_R * collected;
+            const double w_p = w > T_w ? w : 0.0;
+            const double u_p = u > T_u ? u : 0.0;
             const double mean = count / (double)R;
-            const double dw = P.Eps_w * mean - P.g_w * (w - P.B_w) - u_p * P.S_uw;
-            const double du = P.Eps_u * ipriv - P.g_u * (u - P.B_u) - w_p * P.S_wu;
+            const double dw = Eps_w * mean - g_w * (w - B_w) - u_p * S_uw;
+            const double du = Eps_u * ipriv - g_u * (u - B_u) - w_p * S_wu;
             w = w + dw;
             u = u + du;
-            if (w > P.w_max) w = P.w_max;
-            if (w < -P.w_max) w = -P.w_max;
-            if (u > P.u_max) u = P.u_max;
-            if (u < -P.u_max) u = -P.u_max;
-            int over = S.over[k], mode = S.mode[k];
+            if (w > w_max) w = w_max;
+            if (w < -w_max) w = -w_max;
+            if (u > u_max) u = u_max;
+            if (u < -u_max) u = -u_max;
             double dvel = 0.0, dth = 0.0;
             int slot = 2;   // speculative heading: 2 = no turn, 0 / 1 = walk, -1 = relocation
-            const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
-            const double* spec = S.spec + 12 * (k & 1);
+            const bool tw = w > T_w, tu = u > T_w;   // tr_u compares with T_w
             auto walk = [&]() {
-                slot = draws - (int)spec[9];   // the uniform drawn: S.uni[draws]
+                slot = draws - (int)spec_d;   // the uniform drawn: S.uni[draws]
                 ++draws;
-                dvel = P.exp_vel_max;
+                dvel = vmax;
                 over = -1; mode = ABM_MODE_EXPLORE;
             };
             auto reloc = [&]() {
                 const double lm = cleft / (double)nl;
                 const double rm = (count - cleft) / (double)(R - nl);
-                dvel = P.exp_vel_max - vel0;
-                dth = (lm - rm) * P.reloc_theta_max;
+                dvel = vmax - vel0;
+                dth = (lm - rm) * rtmax;
                 slot = -1;
                 over = -1; mode = ABM_MODE_RELOCATE;
             };
             auto exploit = [&]() {
                 over = ABM_MODE_EXPLOIT; mode = ABM_MODE_EXPLOIT;
-                dvel = -vel0 * P.exp_stop_ratio;
+                dvel = -vel0 * stop;
                 dth = 0.0;
             };
             if (over != ABM_MODE_COLLIDE) {   // get_mode() == "collide" only via the override
                 if (!tw && !tu) walk();
-                else if (tw && tu) { if (S.env[k] == 1) exploit(); else reloc(); }
+                else if (tw && tu) { if (env == 1) exploit(); else reloc(); }
                 else if (tw && !tu) reloc();
-                else { if (S.env[k] == 1) exploit(); else walk(); }
+                else { if (env == 1) exploit(); else walk(); }
             }
-            Pose p{S.x[k], S.y[k], S.th[k], vel0};
+            Pose p{x0, y0, th0, vel0};
             double sn, cs;
             if (slot < 0) {
                 p.theta = prove_orientation(p.theta + dth);
@@ -957,13 +972,13 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
             }
             p.v = p.v + dvel;
             // prove_velocity (agent.py:612-620): only in explore mode
-            if (get_mode(over, w, P.T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = P.exp_vel_max;
+            if (get_mode(over, w, T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = vmax;
             p.x = p.x + p.v * cs;
             p.y = p.y - p.v * sn;
-            p = reflect_walls(p, S.rad[k], P.pad, P.width, P.height);
+            p = reflect_walls(p, rad, pad, width, height);
             S.x[k] = p.x; S.y[k] = p.y; S.th[k] = p.theta; S.v[k] = p.v;
             S.w[k] = w; S.u[k] = u; S.over[k] = over; S.mode[k] = mode;
-            S.crb[k] = S.cr[k];
+            S.crb[k] = cr;
             S.nov[k] = ipriv;   // read only by agent k's decision: now its I_priv
             S.spec[24 + (f1 & 1)] = (double)draws;   // uniforms drawn before agent k+1
         }
