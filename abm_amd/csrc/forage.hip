@@ -578,6 +578,7 @@ __device__ inline void fg_wave_sync() {
 // Raw-field word w of one social interval [s, e] of a record: the pieces
 // paint_bits ORs (agent.py:577-590).
 __device__ inline uint32_t interval_word(int w, int s, int e, int R) {
+    if (s >= 0 && e < R) return word_slice_mask(w, s, e > s ? e : s);   // one piece: py_slice(s, e)
     uint32_t bits = 0u;
     int lo, hi;
     if (s < 0) {
