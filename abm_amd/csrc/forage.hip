@@ -1186,7 +1186,17 @@ __device__ inline void notify(const abm_forage_state& st, int32_t* pool_success,
     const int before = st.env_status[g];
     st.env_status[g] = status;
     double* nov = st.novelty + (int64_t)g * tau;
-    for (int q = tau - 1; q > 0; --q) nov[q] = nov[q - 1];   // np.roll(novelty, 1)
+    // np.roll(novelty, 1), eight entries at a time from the top: a chunk's
+    // loads are all issued before its stores, and read below what it writes
+    for (int top = tau - 1; top > 0; top -= 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (top - i > 0) v[i] = nov[top - i - 1];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (top - i > 0) nov[top - i] = v[i];
+    }
     nov[0] = (status - before) > 0 ? 1.0 : 0.0;
     pool_success[g] = 1;
     st.exploited_patch_id[g] = patch_id;
@@ -1336,6 +1346,16 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
     }
     if (phase == ABM_WORLD_STEP) {
         const int np_ = n_live;
+        // the live patches' geometry, quality and units staged in LDS, so
+        // thread 0's walk over the groups below waits on no global load per patch
+        __shared__ double s_pr[kMaxPatches], s_px[kMaxPatches], s_py[kMaxPatches];
+        __shared__ double s_pq[kMaxPatches], s_pl[kMaxPatches];
+        __shared__ int32_t s_pid[kMaxPatches];
+        for (int p = tid; p < np_; p += nthr) {
+            const int64_t s = po + p;
+            s_pr[p] = ps.radius[s]; s_px[p] = ps.x[s]; s_py[p] = ps.y[s];
+            s_pq[p] = ps.quality[s]; s_pl[p] = ps.left[s]; s_pid[p] = ps.id[s];
+        }
         // ---- membership: supcalc.distance(resc, agent) < resc.radius
         const int total = np_ * C * 64;
         for (int base = tid & ~63; base < total; base += nthr) {
@@ -1360,9 +1380,9 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
             int n_kill = 0;
             for (int p = 0; p < np_; ++p) {
                 const int64_t s = po + p;
-                const double pr = ps.radius[s];
-                const double pcx = ps.x[s] + pr, pcy = ps.y[s] + pr;   // resc.center
-                const int32_t pid = ps.id[s];
+                const double pr = s_pr[p];
+                const double pcx = s_px[p] + pr, pcy = s_py[p] + pr;   // resc.center
+                const int32_t pid = s_pid[p];
                 bool destroy = false;
                 for (int c = 0; c < C; ++c) {
                     uint64_t m = member[p * C + c];
@@ -1383,18 +1403,19 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
                         if (pooled(mode, pool_success[g], W.pooling_time)) {
                             notify(st, pool_success, P.tau, g, 1, pid);
                             if (W.teleport) {
-                                st.x[g] = (ps.x[s] + pr) - ra;
-                                st.y[g] = (ps.y[s] + pr) - ra;
+                                st.x[g] = (s_px[p] + pr) - ra;
+                                st.y[g] = (s_py[p] + pr) - ra;
                             }
                         }
                         if (mode == ABM_MODE_EXPLOIT) {
                             // Rescource.deplete (rescource.py:118-133)
                             double units = W.consumption;
-                            const double q = ps.quality[s];
+                            const double q = s_pq[p];
                             if (units > q) units = q;
-                            double left = ps.left[s], dep;
+                            double left = s_pl[p], dep;
                             if (left >= units) { left = left - units; dep = units; }
                             else { dep = left; left = 0.0; }
+                            s_pl[p] = left;
                             ps.left[s] = left;
                             destroy = !(left > 0.0);
                             const double cr = st.collected_r[g];
