@@ -1324,8 +1324,18 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
     __shared__ uint32_t mkey[kMtN];
     __shared__ int s_kind, s_pending, s_pos;
     const int b = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
-    const FGRep P = reps[b];
-    const WRep W = wreps[b];
+    // the replica's parameters in LDS, read where used (held in scalar
+    // registers over thread 0's serial walk they spill)
+    __shared__ FGRep sP;
+    __shared__ WRep sW;
+    static_assert(sizeof(WRep) % 8 == 0 && sizeof(WRep) / 8 <= 64, "WRep copy");
+    if (tid < (int)(sizeof(FGRep) / 8))
+        reinterpret_cast<uint64_t*>(&sP)[tid] = reinterpret_cast<const uint64_t*>(reps + b)[tid];
+    else if (tid >= 64 && tid < 64 + (int)(sizeof(WRep) / 8))
+        reinterpret_cast<uint64_t*>(&sW)[tid - 64] = reinterpret_cast<const uint64_t*>(wreps + b)[tid - 64];
+    __syncthreads();
+    const FGRep& P = sP;
+    const WRep& W = sW;
     const int N = P.n_agents, a0 = P.agent_offset, C = (N + 63) / 64;
     const int64_t po = W.patch_offset;
     int kind = pend[2 * b], n_pending = pend[2 * b + 1];
