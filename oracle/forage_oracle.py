@@ -33,6 +33,8 @@ from collections import OrderedDict
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
+import math
+
 import numpy as np
 
 from .vf_oracle import find_nearest, angle_between, reflect_from_walls, prove_orientation
@@ -522,9 +524,12 @@ class ForageWorld:
                 continue
             destroy = False
             for a in here:
-                x1, y1 = a.position + a.radius                 # bias_agent_towards_res_center
+                # bias_agent_towards_res_center (sims.py:544-552): math.atan2,
+                # i.e. the C library's, not numpy's arctan2 (they differ in
+                # the last bit for a few percent of arguments)
+                x1, y1 = a.position + a.radius
                 x2, y2 = p.center
-                cl = (np.arctan2(y2 - y1, x2 - x1) + a.orientation) % (np.pi * 2)
+                cl = (math.atan2(y2 - y1, x2 - x1) + a.orientation) % (np.pi * 2)
                 a.orientation += (cl - np.pi) * 0.02
                 if destroy:
                     notify_agent(a, -1)

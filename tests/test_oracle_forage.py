@@ -313,3 +313,229 @@ def test_world_single_exploiter_known_answers():
     assert w.patches[0].id == 1 and w.patches[0].resc_left == w.patches[0].resc_units
     assert ag.env_status == -1 and ag.exploited_patch_id == -1
     assert ag.novelty[0] == 0 and ag.novelty[1] == 0
+
+
+# ------------------------------------------- private information (I_priv)
+def test_calc_I_priv_weights_max_novelty_and_new_units():
+    # agent.py:168-175: I_priv = F_N * max(novelty) + F_R * (collected_r -
+    # collected_r_before).  Two novelty entries are 1, so max (1) differs from
+    # sum (2) and mean (0.2); F_N != F_R, so swapped weights show.
+    ag = agent(params=fo.ForageParams(F_N=2.0, F_R=0.5))
+    ag.novelty = np.array([0, 1, 0, 1, 0, 0, 0, 0, 0, 0], dtype=float)
+    ag.collected_r, ag.collected_r_before = 5.5, 3.0
+    fo.calc_I_priv(ag)
+    assert ag.I_priv == 2.0 * 1 + 0.5 * 2.5          # 3.25
+    ag.novelty[:] = 0
+    fo.calc_I_priv(ag)
+    assert ag.I_priv == 1.25                          # only the new units count
+
+
+def test_I_priv_feeds_u_and_resets_collected_before():
+    # one Agent.update: I_priv from the novelty ring and the units collected
+    # since the last step, du = Eps_u*I_priv - g_u*(u - B_u) - w_p*S_wu
+    # (agent.py:201), then collected_r_before = collected_r (agent.py:283)
+    p = fo.ForageParams(F_N=1.0, F_R=0.25, Eps_u=0.5, g_u=0.1, B_u=0.0, Eps_w=0.0, g_w=0.0,
+                        T_w=0.5, T_u=0.5, u_max=10.0)
+    ag = agent(params=p, pos=(200.0, 200.0))
+    ag.novelty = np.array([0, 0, 1, 0, 0, 0, 0, 0, 0, 0], dtype=float)
+    ag.collected_r, ag.collected_r_before = 4.0, 2.0
+    ag.u, ag.w = 0.2, 0.0
+    ag.set_mode("collide")                            # no draw, no motion
+    fo.agent_update(ag, [ag])
+    assert ag.I_priv == 1.0 + 0.25 * 2.0             # 1.5
+    # w_p = 0 (w = 0 < T_w): du = 0.5*1.5 - 0.1*0.2 = 0.73
+    assert ag.u == 0.2 + (0.5 * 1.5 - 0.1 * (0.2 - 0.0) - 0 * p.S_wu)
+    assert math.isclose(ag.u, 0.93, rel_tol=1e-15)
+    assert ag.collected_r_before == 4.0
+
+
+# ------------------------------------------- keep_distance_info amplitude
+def _kd_agent(vision_range):
+    return agent(pos=(10.0, 10.0), th=0.0, R=16, vision_range=vision_range)
+
+
+def test_keep_distance_uses_the_last_loop_distance():
+    # agent.py:587-590 weighs every main slice with 1 - distance/vision_range,
+    # where `distance` is the loop variable left by the LAST obstacle the loop
+    # measured (agent.py:528), visible or not.  Obstacle A at (10, 12): d = 2,
+    # rays [2, 5) -> output {11, 12, 13} (see test_truncated_half_width_side_object).
+    # Obstacle B at (6, 10) is straight behind (a' = pi -> invisible), d = 4.
+    # Order [A, B]: amplitude 1 - 4/10 = 0.6; order [B, A]: 1 - 2/10 = 0.8.
+    a, b = Obj((10.0, 12.0)), Obj((6.0, 10.0))
+    f = fo.projection_field(_kd_agent(10.0), [a, b], keep_distance_info=True)
+    assert out_bits(f) == {11, 12, 13} and np.all(f[[11, 12, 13]] == 1 - 4.0 / 10.0)
+    f = fo.projection_field(_kd_agent(10.0), [b, a], keep_distance_info=True)
+    assert out_bits(f) == {11, 12, 13} and np.all(f[[11, 12, 13]] == 1 - 2.0 / 10.0)
+
+
+def test_keep_distance_leaves_the_wrapped_piece_at_one():
+    # angle 3.1 with cross > 0 -> ca = -3.1 -> t = 0 (phis[0] = -pi is nearest);
+    # d = 2 -> ps/2 = 1.18 -> start = int(-1.18) = -1, end = int(1.18) = 1.
+    # start < 0 paints V[15:16] = 1 (not the amplitude, agent.py:579-581),
+    # then the main slice V[0:1] = 1 - d/vision_range.  Flipped: out[0] = 1,
+    # out[15] = 1 - d/10.
+    a = 3.1
+    ob = Obj((10.0 + 2 * math.cos(a), 10.0 + 2 * math.sin(a)))
+    f = fo.projection_field(_kd_agent(10.0), [ob], keep_distance_info=True)
+    assert out_bits(f) == {0, 15}
+    assert f[0] == 1.0
+    assert math.isclose(f[15], 0.8, rel_tol=0, abs_tol=1e-15)
+
+
+# ------------------------------------------- vision range (own radii)
+@pytest.mark.parametrize("x,visible", [(58.0, True), (58.5, False), (47.0, True)])
+def test_vision_range_uses_each_agents_own_radius(x, visible):
+    # supcalc.distance (supcalc.py:73-78) measures centres with each agent's
+    # OWN radius: focal radius 10 at (0, 0) -> centre (10, 10); cue radius 2
+    # at (x, 8) -> centre (x + 2, 10): d = x - 8.  vision_range 50 and the
+    # test is <= (agent.py:400): x = 58 -> d = 50 exactly -> in range;
+    # x = 58.5 -> out.  With the focal radius for both centres d would be x
+    # (58 -> out).  The cue exploits patch 5, the focal none.
+    ags = [agent(pos=(0.0, 0.0), aid=0, R=64, radius=10, vision_range=50.0),
+           agent(pos=(x, 8.0), aid=1, R=64, radius=2, vision_range=50.0)]
+    ags[1].set_mode("exploit")
+    ags[1].exploited_patch_id = 5
+    fo.calc_social_V_proj(ags[0], ags)
+    assert bool(ags[0].soc_v_field.any()) == visible
+
+
+# ------------------------------------------- resource loop branches
+def _one_patch_world(ag_list, **kw):
+    np.random.seed(3)
+    w = _world(ag_list, n_resc=1, lo=50, hi=60, **kw)
+    p = w.patches[0]
+    p.position = np.array([200.0, 200.0])
+    p.center = (230.0, 230.0)
+    return w, p
+
+
+def test_pooling_time_notifies_only_pooled_relocators():
+    # sims.py:816-822, 847-853 with pooling_time > 0: on a patch only agents
+    # in "pool"/"relocate" with pool_success are told 1 (and the patch id);
+    # off the patch those are told -1, and so are exploiters; an explorer is
+    # left alone.  pool_success and the env status are touched only by
+    # notify_agent, which the update after the loop never calls.
+    ags = _agents(5, N=6)
+    for a in ags:
+        a.pooling_time = 2
+        a.is_moved_with_cursor = 1
+        a.novelty = np.zeros(a.novelty.shape)
+    w, p = _one_patch_world(ags)
+    on = lambda a: a.__setattr__("position", np.array([225.0, 222.0]))   # centre (235, 232)
+    # 0: relocating, pooled -> told 1 on the patch
+    on(ags[0]); ags[0].w = 0.9; ags[0].pool_success = 1
+    # 1: relocating, not pooled -> nothing
+    on(ags[1]); ags[1].w = 0.9; ags[1].pool_success = 0
+    # 2: exploring on the patch -> nothing
+    on(ags[2]); ags[2].w = 0.0; ags[2].pool_success = 1
+    # 3: off the patch, relocating and pooled -> told -1
+    ags[3].position = np.array([40.0, 40.0]); ags[3].w = 0.9; ags[3].pool_success = 1
+    # 4: off the patch, exploiting -> told -1
+    ags[4].position = np.array([40.0, 80.0]); ags[4].set_mode("exploit"); ags[4].exploited_patch_id = 1
+    ags[4].env_status = 1
+    # 5: off the patch, exploring -> nothing
+    ags[5].position = np.array([80.0, 40.0]); ags[5].w = 0.0; ags[5].pool_success = 0
+    ags[5].env_status = 1
+    w.step()
+    assert (ags[0].env_status, ags[0].exploited_patch_id, ags[0].novelty[0]) == (1, p.id, 1)
+    assert (ags[1].env_status, ags[1].exploited_patch_id, ags[1].pool_success) == (0, -1, 0)
+    assert (ags[2].env_status, ags[2].novelty[0]) == (0, 0)
+    assert (ags[3].env_status, ags[3].exploited_patch_id) == (-1, -1)
+    assert (ags[4].env_status, ags[4].exploited_patch_id) == (-1, -1)
+    assert ags[5].env_status == 1 and ags[5].pool_success == 0
+
+
+def test_teleport_and_bias_towards_patch_centre():
+    # sims.py:544-552: cl = (atan2(dy, dx) + theta) % 2pi with (dx, dy) from
+    # the agent centre to the patch centre, theta += (cl - pi) * 0.02; then
+    # notify and, with TELEPORT_TO_MIDDLE, position = patch position + patch
+    # radius - agent radius (sims.py:823-824).
+    # Agent centre 5 px left of the patch centre: dx = 5, dy = 0, atan2 = 0;
+    # theta = 0 -> cl = 0 -> theta = -0.02 pi.  A second agent 5 px below
+    # (dy = -5, atan2 = -pi/2) heading pi/2: cl = 0 -> pi/2 - 0.02 pi; a third
+    # 5 px above (dy = 5, atan2 = pi/2) heading pi/2: cl = pi -> no turn.
+    ags = _agents(6, N=3)
+    for a in ags:
+        a.is_moved_with_cursor = 1
+    w, p = _one_patch_world(ags, teleport_exploit=True)
+    ags[0].position = np.array([215.0, 220.0]); ags[0].orientation = 0.0
+    ags[1].position = np.array([220.0, 225.0]); ags[1].orientation = PI / 2
+    ags[2].position = np.array([220.0, 215.0]); ags[2].orientation = PI / 2
+    w.step()
+    assert ags[0].orientation == (0.0 - PI) * 0.02
+    assert ags[1].orientation == PI / 2 + (0.0 - PI) * 0.02
+    assert ags[2].orientation == PI / 2
+    for a in ags:
+        assert np.array_equal(a.position, [220.0, 220.0])      # 200 + 30 - 10
+        assert a.env_status == 1 and a.exploited_patch_id == p.id
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_patch_placement_ranges_and_draw_order(overlap):
+    # sims.py:349-359: x, y = randint over [pad - r, W + pad - r) with border
+    # overlap, else [pad, W + pad - 2r); then units = randint(min, max) and
+    # quality = uniform(min_q, max_q), in that order, from the global stream.
+    W, r, pad = 300, 30, 30
+    xs = []
+    for seed in range(40):
+        np.random.seed(seed)
+        w = fo.ForageWorld([], W, W, pad, 1, r, 5, 9, 0.2, 0.7, allow_border_overlap=overlap)
+        g = np.random.RandomState(seed)
+        lo, hi = (pad - r, W + pad - r) if overlap else (pad, W + pad - 2 * r)
+        x, y = g.randint(lo, hi), g.randint(lo, hi)
+        units, q = g.randint(5, 9), g.uniform(0.2, 0.7)
+        p = w.patches[0]
+        assert (p.position[0], p.position[1], p.resc_units, p.unit_per_timestep) == (x, y, units, q)
+        xs += [x, y]
+    if overlap:
+        assert min(xs) < pad           # reaches over the left/top border
+    else:
+        assert min(xs) >= pad and max(xs) <= W + pad - 2 * r - 1
+
+
+# ------------------------------------------- wall reflection (8 branches)
+def _wall_agent(pos, th):
+    a = agent(pos=pos, th=th, radius=1)
+    return a
+
+
+# (position, heading, expected heading, expected position).  The arena is
+# [30, 530]^2 with radius 1.  Each expected heading is the mirror image of
+# the heading at that wall, except at the branch edges where the reference's
+# quadrant bounds decide (agent.py:356-394).
+WALL_CASES = [
+    # left wall: pi/2 <= th < pi -> th - pi/2; pi <= th <= 3pi/2 -> th + pi/2
+    ((20.0, 200.0), 3 * PI / 4, PI / 4, (29.0, 200.0)),
+    ((20.0, 200.0), 5 * PI / 4, 7 * PI / 4, (29.0, 200.0)),
+    ((20.0, 200.0), PI, 3 * PI / 2, (29.0, 200.0)),           # edge: second branch
+    ((20.0, 200.0), PI / 4, PI / 4, (29.0, 200.0)),           # moving away: unchanged
+    # right wall: 3pi/2 <= th < 2pi -> th - pi/2; 0 <= th <= pi/2 -> th + pi/2
+    ((540.0, 200.0), 7 * PI / 4, 5 * PI / 4, (528.0, 200.0)),
+    ((540.0, 200.0), PI / 4, 3 * PI / 4, (528.0, 200.0)),
+    ((540.0, 200.0), PI / 2, PI, (528.0, 200.0)),             # edge: second branch
+    # top wall: pi/2 <= th <= pi -> th + pi/2; 0 <= th < pi/2 -> th - pi/2 (+2pi)
+    ((200.0, 20.0), 3 * PI / 4, 5 * PI / 4, (200.0, 29.0)),
+    ((200.0, 20.0), PI / 4, 7 * PI / 4, (200.0, 29.0)),
+    ((200.0, 20.0), PI / 2, PI, (200.0, 29.0)),               # edge: first branch
+    # bottom wall: 3pi/2 <= th <= 2pi -> th + pi/2 (-2pi); pi <= th < 3pi/2 -> th - pi/2
+    ((200.0, 540.0), 7 * PI / 4, PI / 4, (200.0, 528.0)),
+    ((200.0, 540.0), 5 * PI / 4, 3 * PI / 4, (200.0, 528.0)),
+    ((200.0, 540.0), 3 * PI / 2, 2 * PI, (200.0, 528.0)),     # edge: 2pi is kept (not > 2pi)
+]
+
+
+@pytest.mark.parametrize("pos,th,exp_th,exp_pos", WALL_CASES)
+def test_reflect_from_walls_branches(pos, th, exp_th, exp_pos):
+    a = _wall_agent(pos, th)
+    fo.reflect_from_walls(a)
+    assert tuple(a.position) == exp_pos
+    assert math.isclose(a.orientation, exp_th, rel_tol=0, abs_tol=1e-12)
+
+
+def test_reflect_corner_applies_both_walls():
+    # left and top at once, heading up-left (3pi/4): the left wall turns it to
+    # pi/4 (up-right), then the top wall (0 <= pi/4 < pi/2) to -pi/4 -> 7pi/4
+    a = _wall_agent((20.0, 20.0), 3 * PI / 4)
+    fo.reflect_from_walls(a)
+    assert tuple(a.position) == (29.0, 29.0)
+    assert math.isclose(a.orientation, 7 * PI / 4, abs_tol=1e-12)

@@ -314,3 +314,33 @@ def test_sequential_order_differs_from_synchronous():
     syn = vo.step_arrays(x, y, th, v, 5, 256, (-PI, PI), 400, 400, 30, p, order="synchronous")
     assert np.array_equal(seq[4][0], syn[4][0])
     assert not np.array_equal(seq[4][1], syn[4][1])
+
+
+# ------------------------------------------- aggregation (vf_agent.py:216-237)
+def _rec(aid, pos, th=0.0, R=16):
+    return vo.VFAgentRec(id=aid, radius=1, position=np.array(pos, dtype=float), orientation=th,
+                         velocity=0.0, v_field_res=R, FOV=(-PI, PI), WIDTH=400, HEIGHT=400,
+                         window_pad=30, boundary_cond="walls")
+
+
+def test_aggregation_is_an_or_and_filters_by_id():
+    # Focal id 0 at (10, 10), heading 0, radius 1, R = 16 (phis step 2pi/15).
+    #  * id 1 at (10, 12): v2 = (0, 2), angle +pi/2 -> ca = -pi/2 -> t = 4;
+    #    vis = 2 atan(1/2) -> s = 2.36, floor(s/2) = 1 -> rays [3, 5).
+    #  * id 2 at angle +2.0, distance 2: ca = -2.0 -> t = 3 -> rays [2, 4).
+    #    Rays 3 is covered twice: the sum is binarised, so it stays 1.
+    #  * id 3 at the focal's own position: skipped by coordinate equality
+    #    (vf_supcalc.py:57-58).
+    #  * a second object with id 0 at (10, 8) would draw rays [10, 12)
+    #    (ca = +pi/2 -> t = 11) but the id filter (vf_agent.py:220) drops it.
+    # Raw rays {2, 3, 4}; soc_v_field is in the flipped order -> {13, 12, 11}.
+    focal = _rec(0, (10.0, 10.0))
+    others = [_rec(1, (10.0, 12.0)), _rec(2, (10.0 + 2 * math.cos(2.0), 10.0 + 2 * math.sin(2.0))),
+              _rec(3, (10.0, 10.0)), _rec(0, (10.0, 8.0))]
+    f = vo.calc_soc_v_proj(focal, [focal] + others)
+    assert set(np.flatnonzero(f)) == {11, 12, 13}
+    assert f.max() == 1.0
+    # the same object under another id is drawn
+    others[3] = _rec(4, (10.0, 8.0))
+    f = vo.calc_soc_v_proj(focal, [focal] + others)
+    assert set(np.flatnonzero(f)) == {11, 12, 13} | {15 - 10, 15 - 11}
