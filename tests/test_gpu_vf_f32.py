@@ -48,7 +48,8 @@ def pose_err(g, o, cfg):
                np.max(np.abs(np.asarray(gv) - ov) / np.maximum(np.abs(ov), cfg.V0)))
 
 
-def per_agent_check(sw, b, cfg, before, after, radius=5, overrides=None, max_agents=None):
+def per_agent_check(sw, b, cfg, before, after, radius=5, overrides=None, max_agents=None,
+                    agents=None):
     """Each agent k against the oracle on the poses it saw."""
     bits = sw.fields_numpy(b)
     diag = sw.diag_numpy()[sw.replica_slice(b)]
@@ -58,7 +59,9 @@ def per_agent_check(sw, b, cfg, before, after, radius=5, overrides=None, max_age
     p = vfparams(cfg)
     worst = 0.0
     ov = overrides or {}
-    for k in range(N if max_agents is None else min(N, max_agents)):
+    if agents is None:
+        agents = range(N if max_agents is None else min(N, max_agents))
+    for k in agents:
         vx = np.concatenate([x1[:k], x0[k:]])
         vy = np.concatenate([y1[:k], y0[k:]])
         vt = np.concatenate([t1[:k], t0[k:]])

@@ -223,60 +223,173 @@ def test_world_rejects_bad_patch_count(gpu):
         ResourceWorld(sw, [WorldConfig(N_resc=2, patch_radius=120)])   # randint(30, -10)
 
 
-def test_world_free_run_collected_resource(gpu):
-    # C1 shape (N=10, radius 10, 10 patches of radius 30, quality U[0.5, 1],
-    # 500x500 walls, R=1200, the reference .env decision values) run free for
-    # 100 steps on 12 replicas, device and oracle from the same start and
-    # streams, no state reloaded.  Every replica's collected resource and
-    # RNG stream position must agree; should a replica's trajectory drift
-    # (atan2 of the bias may differ in the last ulp), the ensemble of
-    # collected resource must still agree (Welch t-test at 99.9 %).
-    from scipy import stats
+C1_PARAMS = dict(T_w=0.5, Eps_w=3.0, g_w=0.085, B_w=0.0, w_max=1.0, T_u=0.5, Eps_u=1.0,
+                 g_u=0.085, B_u=0.0, u_max=1.0, S_wu=0.25, S_uw=0.01, Tau=10, F_N=0.5,
+                 F_R=0.5, exp_vel_max=3.0, exp_theta_min=-0.5, exp_theta_max=0.5,
+                 reloc_theta_max=1.8, exp_stop_ratio=0.175)
+C1_WORLD = dict(N_resc=10, patch_radius=30, min_resc_perpatch=200, max_resc_perpatch=1000,
+                min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+
+
+def c1_replicas(B, seed0=4000):
+    """B replicas of the reference's CPU case (N=10, radius 10, R=1200, 10
+    patches of radius 30 with quality U[0.5, 1], 500x500 walls, the .env
+    decision values), created as Simulation.create_agents does
+    (sims.py:526-532), plus matching device state and generators."""
     from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
-    p = fo.ForageParams(T_w=0.5, Eps_w=3.0, g_w=0.085, B_w=0.0, w_max=1.0, T_u=0.5, Eps_u=1.0,
-                        g_u=0.085, B_u=0.0, u_max=1.0, S_wu=0.25, S_uw=0.01, Tau=10, F_N=0.5,
-                        F_R=0.5, exp_vel_max=3.0, exp_theta_min=-0.5, exp_theta_max=0.5,
-                        reloc_theta_max=1.8, exp_stop_ratio=0.175)
-    B, N, R, W, steps = 12, 10, 1200, 500, 100
-    wk = dict(N_resc=10, patch_radius=30, min_resc_perpatch=200, max_resc_perpatch=1000,
-              min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+    p = fo.ForageParams(**C1_PARAMS)
+    N, R, W = 10, 1200, 500
     worlds, allags, rngs, orngs = [], [], [], []
     for b in range(B):
-        g = np.random.RandomState(4000 + b)
+        g = np.random.RandomState(seed0 + b)
         ags = []
-        for i in range(N):            # Simulation.create_agents (sims.py:526-532)
+        for i in range(N):
             x = g.randint(30 - 10, W + 30 - 10)
             y = g.randint(30 - 10, W + 30 - 10)
             ags.append(fo.ForageAgent(i, 10, (x, y), g.uniform(0, 2 * PI), R, (-PI, PI), W, W, 30,
                                       2000.0, False, True, p, pooling_time=0))
         worlds.append(ags)
         allags += ags
-        rngs.append(np.random.RandomState())
-        rngs[-1].set_state(g.get_state())
-        orngs.append(np.random.RandomState())
-        orngs[-1].set_state(g.get_state())
+        for lst in (rngs, orngs):
+            lst.append(np.random.RandomState())
+            lst[-1].set_state(g.get_state())
     cfg = forage_config(N, W, R, p)
     cfg.vision_range = 2000.0
     sw = ForageSwarm([cfg] * B, [a.position[0] for a in allags], [a.position[1] for a in allags],
                      [a.orientation for a in allags], radius=10, rngs=rngs)
-    rw = ResourceWorld(sw, [WorldConfig(**wk)] * B)
+    rw = ResourceWorld(sw, [WorldConfig(**C1_WORLD)] * B)
     oracle = []
     for b in range(B):
         np.random.set_state(orngs[b].get_state())
-        oracle.append(oracle_world(worlds[b], W, wk))
+        oracle.append(oracle_world(worlds[b], W, C1_WORLD))
         orngs[b].set_state(np.random.get_state())
     rw.create_resources()
+    return sw, rw, oracle, rngs, orngs
+
+
+def oracle_step(world, g):
+    np.random.set_state(g.get_state())
+    world.step()
+    g.set_state(np.random.get_state())
+
+
+DISCRETE = ("mode", "overriding_mode", "env_status", "exploited_patch_id")
+
+
+def test_world_c1_one_step_parity_over_1000_steps(gpu):
+    # The C1 run for its full 1000 steps along the oracle's trajectory: every
+    # step starts the device from the oracle's state (agents and generator),
+    # so every step's discrete state must be identical and the continuous
+    # state within 1e-12.  Patches are never reloaded: they must stay
+    # identical on their own.
+    B, steps = 12, 1000
+    sw, rw, oracle, rngs, orngs = c1_replicas(B)
+    for b in range(B):
+        assert same_stream(rngs[b], orngs[b])
+        compare(sw, rw, oracle[b], b, ("create", b))
+    print(f"C1 one-step parity: {B} replicas x {steps} steps", flush=True)
     for step in range(steps):
         rw.step()
         for b in range(B):
-            np.random.set_state(orngs[b].get_state())
-            oracle[b].step()
-            orngs[b].set_state(np.random.get_state())
-    got = sw.get_state("collected_r")["collected_r"].reshape(B, N).sum(axis=1)
+            oracle_step(oracle[b], orngs[b])
+        st = sw.get_state()
+        for b in range(B):
+            assert same_stream(rngs[b], orngs[b]), (step, b)
+            sl = sw.replica_slice(b)
+            arr = fo.agents_arrays(oracle[b].agents)
+            for k in DISCRETE + ("collected_r",):
+                assert np.array_equal(st[k][sl], arr[k]), (step, b, k)
+            for k in ("x", "y", "theta", "v", "w", "u", "I_priv", "collected_r_before"):
+                assert rel(st[k][sl], arr[k]) <= 1e-12, (step, b, k)
+            assert np.array_equal(st["novelty"].reshape(-1, sw.tau)[sl], arr["novelty"])
+            got = rw.patches(b) if step % 50 == 0 else None
+            if got is not None:
+                assert np.array_equal(got["id"], [q.id for q in oracle[b].patches])
+                assert np.array_equal(got["left"], [q.resc_left for q in oracle[b].patches])
+            # the next step starts from the oracle's continuous values
+            for k in ("x", "y", "theta", "v", "w", "u", "I_priv", "collected_r_before"):
+                st[k][sl] = arr[k]
+        sw.set_state(**{k: st[k] for k in ("x", "y", "theta", "v", "w", "u", "I_priv",
+                                           "collected_r_before")})
+        if step % 200 == 0:
+            print(f"  step {step} ok", flush=True)
+    assert sum(a.collected_r for w in oracle for a in w.agents) > 0
+
+
+def test_world_c1_free_run(gpu):
+    # C1 run free for its full 1000 steps on device and oracle from the same
+    # start and streams, nothing reloaded.  The device's libm and the host's
+    # (glibc, numpy) differ in the last bit of atan2 / cos / sin for about
+    # one call in a thousand, and the reference's bias towards the patch
+    # centre multiplies a heading difference by 1.02 per step an agent stays
+    # on a patch (sims.py:551-552; test_bias_multiplies_heading_differences_
+    # by_1_02_per_step), so a last-bit difference grows about tenfold per 100
+    # steps.  Bar: every replica's discrete state and generator position
+    # identical for the first 600 steps (the growth needs about 800 steps
+    # from one ulp to a changed decision), and the ensemble of collected
+    # resource after 1000 steps equal by Welch's t-test at 99.9 %.
+    from scipy import stats
+    B, steps, exact_until = 12, 1000, 600
+    sw, rw, oracle, rngs, orngs = c1_replicas(B)
+    first_diff = [None] * B
+    for step in range(steps):
+        rw.step()
+        for b in range(B):
+            oracle_step(oracle[b], orngs[b])
+        if step < exact_until or step % 50 == 0:
+            st = sw.get_state(*DISCRETE, "collected_r")
+            for b in range(B):
+                if first_diff[b] is not None:
+                    continue
+                sl = sw.replica_slice(b)
+                arr = fo.agents_arrays(oracle[b].agents)
+                if not (same_stream(rngs[b], orngs[b]) and all(
+                        np.array_equal(st[k][sl], arr[k]) for k in DISCRETE + ("collected_r",))):
+                    first_diff[b] = step
+    got = sw.get_state("collected_r")["collected_r"].reshape(B, 10).sum(axis=1)
     exp = np.array([sum(a.collected_r for a in oracle[b].agents) for b in range(B)])
-    same = [b for b in range(B) if same_stream(rngs[b], orngs[b]) and got[b] == exp[b]]
-    print(f"collected resource: device {got.mean():.2f} +- {got.std():.2f}, oracle "
-          f"{exp.mean():.2f} +- {exp.std():.2f}; {len(same)} of {B} replicas identical", flush=True)
+    print(f"collected resource after {steps} steps: device {got.mean():.2f} +- {got.std():.2f}, "
+          f"oracle {exp.mean():.2f} +- {exp.std():.2f}; first discrete difference per replica "
+          f"{first_diff}", flush=True)
+    assert all(f is None or f >= exact_until for f in first_diff), first_diff
     assert exp.sum() > 0
-    assert stats.ttest_ind(got, exp, equal_var=False).pvalue > 1e-3 or np.array_equal(got, exp)
-    assert len(same) >= B // 2
+    assert np.array_equal(got, exp) or stats.ttest_ind(got, exp, equal_var=False).pvalue > 1e-3
+
+
+def test_world_membership_over_several_words(gpu):
+    # 170 agents: a patch's membership spans three 64-agent words, so the
+    # group walk, notify order and the depletion broadcast cross words.  Six
+    # large patches (radius 60) on a 400 px arena with small stocks, so
+    # patches are emptied and regenerated while crowded; teleport on.
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    N, W, R = 170, 400, 256
+    wk = dict(N_resc=6, patch_radius=60, min_resc_perpatch=3, max_resc_perpatch=9,
+              min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=True)
+    ags, p = make_agents(1170, N, W, R, True, 0)
+    rng, orng = np.random.RandomState(1171), np.random.RandomState(1171)
+    sw = ForageSwarm([forage_config(N, W, R, p)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=10,
+                     rngs=[rng])
+    rw = ResourceWorld(sw, [WorldConfig(**wk)])
+    load_agents(sw, rw, ags, 0)
+    np.random.set_state(orng.get_state())
+    world = oracle_world(ags, W, wk)
+    orng.set_state(np.random.get_state())
+    rw.create_resources()
+    compare(sw, rw, world, 0, "create")
+    kills = 0
+    crowded = 0
+    for step in range(30):
+        rw.step(fields=step % 3 == 0)
+        for q in world.patches:
+            words = {a.id // 64 for a in world.agents if fo.distance(q, a) < q.radius}
+            crowded = max(crowded, len(words))
+        before = {q.id: tuple(q.position) for q in world.patches}
+        np.random.set_state(orng.get_state())
+        world.step()
+        orng.set_state(np.random.get_state())
+        kills += sum(before[q.id] != tuple(q.position) for q in world.patches if q.id in before)
+        assert same_stream(rng, orng), step
+        compare(sw, rw, world, 0, step)
+        load_agents(sw, rw, world.agents, 0)
+    assert kills > 0 and crowded == 3, (kills, crowded)   # a patch held agents of all three words

@@ -539,3 +539,27 @@ def test_reflect_corner_applies_both_walls():
     fo.reflect_from_walls(a)
     assert tuple(a.position) == (29.0, 29.0)
     assert math.isclose(a.orientation, 7 * PI / 4, abs_tol=1e-12)
+
+
+def test_bias_multiplies_heading_differences_by_1_02_per_step():
+    # bias_agent_towards_res_center (sims.py:544-552) is
+    #   theta' = theta + ((atan2(dy, dx) + theta) mod 2pi - pi) * 0.02,
+    # so d theta' / d theta = 1.02 away from the mod wrap: a heading
+    # difference of an agent that stays on a patch grows by 2 % per step.
+    # This is why a last-bit difference of a transcendental function
+    # between two hosts eventually changes discrete state in long runs.
+    # Two copies 1e-9 rad apart, agent parked on the patch (not moved by the
+    # update), 50 steps: the difference is 1e-9 * 1.02^50.
+    thetas = []
+    for d in (0.0, 1e-9):
+        ags = _agents(6, N=1)
+        ags[0].is_moved_with_cursor = 1
+        w, p = _one_patch_world(ags)
+        ags[0].position = np.array([215.0, 220.0])    # centre 5 px left of the patch centre
+        # atan2(0, 5) = 0: the fixed point is theta = pi; start 0.1 above it
+        ags[0].orientation = PI + 0.1 + d
+        for _ in range(50):
+            w.step()
+        thetas.append(ags[0].orientation)
+    assert math.isclose(thetas[1] - thetas[0], 1e-9 * 1.02 ** 50, rel_tol=1e-5)
+    assert math.isclose(thetas[0] - PI, 0.1 * 1.02 ** 50, rel_tol=1e-9)
