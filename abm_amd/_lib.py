@@ -148,6 +148,8 @@ _SIGS = {
     "abm_vf_projection_rows": (C.c_int, [_I32, _D, _D, _I32, _D, _D, _D, _D, _D, _D,
                                          _P, _P, _P, _I32, _P, _P, _P]),
     "abm_vf_state_variables": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P]),
+    "abm_vf_state_variables_noalloc": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P,
+                                                 _P]),
     "abm_forage_batch_create": (C.c_int, [C.POINTER(ForageReplica), _I32, C.POINTER(_P)]),
     "abm_forage_batch_destroy": (C.c_int, [_P]),
     "abm_forage_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),

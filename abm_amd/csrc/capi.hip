@@ -100,6 +100,53 @@ static int probe_tables(int R, ProbeTables* out) {
     return ABM_OK;
 }
 
+// ------------------------------------------------ no-allocation probe
+// abm_vf_state_variables_noalloc keeps the float64 ray tables it needs in
+// this static device array instead of memory the library allocates.  Every
+// GPU the library runs on holds its own copy until the process ends; host
+// code reaches a device's copy only through the address hipGetSymbolAddress
+// returns there (kept per device below), never through the array's name.
+// Every call on a GPU rewrites that GPU's copy, so calls that overlap on one
+// GPU (other streams, other host threads) race on it.
+constexpr int kNoAllocMaxRays = 8192;
+static __device__ __align__(256) double scratch[3 * kNoAllocMaxRays];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+static int noalloc_scratch(double** out) {
+    int dev = 0;
+    if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(ABM_ERR_UNSUPPORTED, "no-allocation entry point: device %d beyond the %d "
+                         "it keeps table space for", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                               "hipGetSymbolAddress(scratch)"))
+            return rc;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ABM_OK;
+}
+
+// cos(PHI_k), sin(PHI_k) and diff(PHI)_k (last 0) of PHI = np.arange(-pi, pi,
+// 2 pi / R), built on the device: the grid is exact (numpy's arange
+// arithmetic), cos and sin are the device's (within an ulp of the host's).
+__global__ void ray_tables_f64_kernel(int R, double delta, double* __restrict__ c,
+                                      double* __restrict__ s, double* __restrict__ d) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= R) return;
+    const double phi = arange_at(k, delta);
+    c[k] = cos(phi);
+    s[k] = sin(phi);
+    d[k] = k + 1 < R ? arange_at(k + 1, delta) - phi : 0.0;
+}
+
 // Replicas that may share a wave of the grouped kernel: every constant the
 // kernel reads from the group's first replica must agree.
 static bool group_class_less(const VFRep& a, const VFRep& b) {
@@ -488,6 +535,28 @@ int abm_vf_state_variables(int32_t R, int32_t n, const uint32_t* bits, const dou
     const int phi_ok = arange_len(R) == R ? 1 : 0;
     return launch_vf_state_variables(R, n, bits, vel, alp0, bet0, v0, gam, alp1, bet1, t.c, t.s, t.d,
                                      phi_ok, out, dphi, (hipStream_t)stream);
+}
+
+int abm_vf_state_variables_noalloc(int32_t R, int32_t n, const uint32_t* bits, const double* vel,
+                                   const double* alp0, const double* bet0, const double* v0,
+                                   double gam, double alp1, double bet1, double* out, int8_t* dphi,
+                                   void* stream) {
+    if (R < 2 || R > kNoAllocMaxRays)
+        return set_error(ABM_ERR_INVALID, "abm_vf_state_variables_noalloc: n_rays=%d outside [2, %d] "
+                         "(its static table space holds %d rays)", R, kNoAllocMaxRays, kNoAllocMaxRays);
+    if (n < 0) return set_error(ABM_ERR_INVALID, "abm_vf_state_variables_noalloc: n < 0");
+    if (n > 0 && (!bits || !vel || !alp0 || !bet0 || !v0 || !out))
+        return set_error(ABM_ERR_INVALID, "abm_vf_state_variables_noalloc: NULL array");
+    if (n == 0) return ABM_OK;
+    double* t = nullptr;
+    if (int rc = noalloc_scratch(&t)) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(ray_tables_f64_kernel, dim3((R + 255) / 256), dim3(256), 0, s, R,
+                       arange_delta(R), t, t + R, t + 2 * R);
+    if (int rc = check_hip(hipGetLastError(), "ray_tables_f64_kernel launch")) return rc;
+    const int phi_ok = arange_len(R) == R ? 1 : 0;
+    return launch_vf_state_variables(R, n, bits, vel, alp0, bet0, v0, gam, alp1, bet1, t, t + R,
+                                     t + 2 * R, phi_ok, out, dphi, s);
 }
 
 }  // extern "C"

@@ -7,7 +7,8 @@
  * last failure on the calling thread is returned by abm_last_error().  No
  * entry point throws across the ABI.  Kernels are launched on the caller's
  * stream (a hipStream_t passed as void*; NULL = the null stream) and are
- * stream ordered; a call allocates nothing except abm_*_create.
+ * stream ordered; a call allocates nothing except abm_*_create and the first
+ * abm_vf_state_variables call for an n_rays (its cached ray tables).
  *
  * Each step entry point replaces one call of the reference's per-step agent
  * update (pygame Group.update over all agents):
@@ -189,6 +190,21 @@ int abm_vf_state_variables(int32_t n_rays, int32_t n, const uint32_t* bits, cons
                            const double* alp0, const double* bet0, const double* v0,
                            double gam, double alp1, double bet1,
                            double* out, int8_t* dphi, void* stream);
+
+/* abm_vf_state_variables for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference: abm_vf_state_variables
+ * allocates its float64 ray tables on first use per R; this variant calls no
+ * allocating routine and keeps the tables in a static device array of the
+ * library (3 * 8192 doubles = 192 KiB per GPU, held until the process ends),
+ * rebuilt by a small kernel on every call.  Same arguments and results as
+ * abm_vf_state_variables (cos/sin of the grid from the device's libm, within
+ * an ulp of the host's); n_rays above 8192 get ABM_ERR_INVALID.  Calls that
+ * overlap on one GPU (other streams or host threads) race on the tables. */
+int abm_vf_state_variables_noalloc(int32_t n_rays, int32_t n, const uint32_t* bits,
+                                   const double* vel, const double* alp0, const double* bet0,
+                                   const double* v0, double gam, double alp1, double bet1,
+                                   double* out, int8_t* dphi, void* stream);
 
 /* ------------------------------------------------ cooperative signalling */
 /* Field parameters of cs_supcalc.projection_field
