@@ -106,6 +106,7 @@ class PatchState(C.Structure):
 
 WORLD_STEP, WORLD_RESUME, WORLD_FILL = 0, 1, 2                       # ABM_WORLD_* phases
 WORLD_DONE, WORLD_NEED_WORDS, WORLD_TIMEOUT, WORLD_BAD_COUNT = 0, 1, 2, 4   # statuses
+FORAGE_PATH_AUTO, FORAGE_PATH_SWEEP = 0, 1                             # ABM_FORAGE_PATH_*
 
 
 class CSParams(C.Structure):
@@ -154,6 +155,8 @@ _SIGS = {
     "abm_forage_batch_destroy": (C.c_int, [_P]),
     "abm_forage_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
     "abm_forage_batch_get_offsets": (C.c_int, [_P, _P, _P]),
+    "abm_forage_set_path": (C.c_int, [_P, _I32]),
+    "abm_forage_sweep_counts": (C.c_int, [_P, _P, _P]),
     "abm_forage_step_f64": (C.c_int, [_P, C.POINTER(ForageState), _P, _P, _P, _P, _P, _P]),
     "abm_forage_projection": (C.c_int, [_I32, _D, _D, _I32, _D, _I32, _D, _D, _D, _D,
                                         _P, _P, _I32, _I32, _P, _P]),

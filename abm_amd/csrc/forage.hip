@@ -338,13 +338,12 @@ __device__ inline FastRec fast_rec_const(int R, double step, double fov0, double
 //    t - floor(half) - 1 when that stays positive, else t - floor(half),
 //    and end = t + floor(half).
 // The float64 distance, which only visual exclusion ranks by, is exact.
-__device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Focal& f, int R, double step,
+__device__ inline Rec record_at(double ox, double oy, int c, int order, const Focal& f, int R, double step,
                                 double fov0, double fov1, const FastRec& FR, bool need_dist) {
 #pragma clang fp contract(off)
     Rec rc;
     rc.valid = 0; rc.social = 0; rc.order = 0; rc.start = rc.end = rc.sx = rc.ex = 0;
     rc.dist = 0.0; rc.vis = 0.0;
-    const double ox = S.x[j], oy = S.y[j];
     if (ox == f.px && oy == f.py) return rc;                 // skipped (agent.py:502)
     const double v2x = (ox + f.radius) - f.cx, v2y = (oy + f.radius) - f.cy;
     {
@@ -370,7 +369,7 @@ __device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Foca
                 const int t = (int)tr, h = (int)hfl;
                 rc.valid = 1;
                 rc.social = c == kSocial;
-                rc.order = c * N + j;
+                rc.order = order;
                 rc.start = rc.sx = t - h >= 1 ? t - h - 1 : t - h;
                 rc.end = rc.ex = t + h;
                 if (need_dist) rc.dist = norm2(v2x, v2y);
@@ -382,13 +381,18 @@ __device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Foca
     if (!p.skipped && p.visible) {
         rc.valid = 1;
         rc.social = c == kSocial;
-        rc.order = c * N + j;
+        rc.order = order;
         rc.start = rc.sx = p.start;
         rc.end = rc.ex = p.end;
         rc.dist = p.dist;
         rc.vis = p.vis;
     }
     return rc;
+}
+
+__device__ inline Rec record_of(const FGSmem& S, int j, int c, int N, const Focal& f, int R, double step,
+                                double fov0, double fov1, const FastRec& FR, bool need_dist) {
+    return record_at(S.x[j], S.y[j], c, c * N + j, f, R, step, fov0, fov1, FR, need_dist);
 }
 
 // Classes of the agents j != skip for focal agent f into S.cls; those whose
@@ -1117,6 +1121,389 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     }
 }
 
+// ------------------------------------------------ replicas beyond one workgroup
+// Agent.update in id order (sims.py:861) for a replica whose agents do not fit
+// one workgroup's LDS, without visual exclusion.  Agent k's update reads the
+// agents j < k after their own update and j > k before it, and only through
+// what makes j a social cue for k: j's overriding mode (exploit), its patch
+// id (never changed by the update) and its position.  Sweeps recompute every
+// agent from the latest tentative states X^m (j < k) and the step's start
+// (j >= k) at once:
+//   X^{m+1}_k = F_k(X^m_0, .., X^m_{k-1}, start_k, .., start_{N-1}),
+// the walk's draw index being the number of draws in X^{m+1}_j, j < k.
+// Agent k is final after k + 1 sweeps, and the only fixed point X = F(X) is
+// the reference's sequential result, so the sweeps stop at the first
+// X^{m+1} == X^m (every value bitwise), which is that result; in practice a
+// few sweeps, as only exploiting agents carry dependencies.
+struct SweepX {
+    double *x, *y, *th, *v, *w, *u, *ip, *dth, *dvel;
+    int32_t *over, *mode, *kind;   // kind: 0 = draws a walk turn, 1 = fixed increments, -1 = unset
+};
+
+// Sweep control words.
+constexpr int kSwConverged = 0, kSwSkip = 1, kSwCand = 2, kSwIters = 3, kSwDraws = 4,
+              kSwUsed0 = 5, kSwPos0 = 6, kSwBad = 7, kSwWords = 8;
+
+constexpr int kSweepMaxWords = 256;   // R <= 8192: four field words per lane
+
+// Exploiting agents with a patch (at the step's start or in X), the only
+// candidates for a social cue.
+__device__ inline void sweep_candidates(const abm_forage_state& st, int a0, int N, const int32_t* xover,
+                                        int32_t* cand, int32_t* ctrl) {
+    if (threadIdx.x == 0) ctrl[kSwCand] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int base = threadIdx.x - lane; base < N; base += blockDim.x) {
+        const int j = base + lane;
+        bool c = false;
+        if (j < N) {
+            const int g = a0 + j;
+            c = st.exploited_patch_id[g] != -1 &&
+                (st.overriding_mode[g] == ABM_MODE_EXPLOIT || (xover && xover[j] == ABM_MODE_EXPLOIT));
+        }
+        const uint64_t m = __ballot(c);
+        if (m) {
+            int at = 0;
+            if (lane == 0) at = atomicAdd(&ctrl[kSwCand], (int)__popcll(m));
+            at = __builtin_amdgcn_readfirstlane(at);
+            if (c) cand[at + (int)__popcll(m & ((1ull << lane) - 1ull))] = j;
+        }
+    }
+    __syncthreads();
+}
+
+// One block of 1024 threads: status, the step's uniforms, X^0 = the start
+// state, the first candidates.
+__global__ void __launch_bounds__(1024)
+sweep_prep_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
+                  const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
+                  const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
+                  const int32_t* __restrict__ u_used, const int32_t* __restrict__ status,
+                  const abm_mt19937_state* __restrict__ mt, SweepX X0, double* __restrict__ uni,
+                  int32_t* __restrict__ cand, int32_t* __restrict__ ctrl) {
+    __shared__ uint32_t key[kMtN];
+    const FGRep& P = reps[rb];
+    const int N = P.n_agents, a0 = P.agent_offset, tid = threadIdx.x, nthr = blockDim.x;
+    const bool skip = status && status[rb] != kWorldReady;
+    if (tid == 0) {
+        ctrl[kSwConverged] = 0; ctrl[kSwSkip] = skip ? 1 : 0; ctrl[kSwIters] = 0;
+        ctrl[kSwDraws] = 0; ctrl[kSwBad] = 0;
+        ctrl[kSwUsed0] = (words || mt) ? u_used[rb] : 0;
+        ctrl[kSwPos0] = mt ? mt[rb].pos : 0;
+    }
+    if (skip) return;   // uniform over the block
+    for (int i = tid; i < N; i += nthr) {
+        const int g = a0 + i;
+        X0.x[i] = st.x[g]; X0.y[i] = st.y[g]; X0.th[i] = st.theta[g]; X0.v[i] = st.v[g];
+        X0.w[i] = st.w[g]; X0.u[i] = st.u[g]; X0.ip[i] = st.I_priv[g];
+        X0.dth[i] = 0.0; X0.dvel[i] = 0.0;
+        X0.over[i] = st.overriding_mode[g]; X0.mode[i] = st.mode[g]; X0.kind[i] = -1;
+    }
+    // agent i's walk draws the i-th uniform at most: doubles, pairs of words
+    // after those the resource loop used, or the device generator's next 2N
+    // words; NaN past the end of a buffer
+    if (mt) {
+        for (int i = tid; i < kMtN; i += nthr) key[i] = mt[rb].key[i];
+        __syncthreads();
+        uint32_t* wb = reinterpret_cast<uint32_t*>(uni);
+        int made = 0, p = mt[rb].pos;
+        while (made < 2 * N) {
+            if (p >= kMtN) {
+                mt_twist_block(key);
+                p = 0;
+            }
+            const int take = min(kMtN - p, 2 * N - made);
+            for (int j = tid; j < take; j += nthr) wb[made + j] = mt_temper(key[p + j]);
+            made += take;
+            p += take;
+            __syncthreads();
+        }
+        for (int i = tid; i < N; i += nthr) {   // each thread rewrites its own 8 bytes
+            const uint32_t w0 = wb[2 * i], w1 = wb[2 * i + 1];
+            uni[i] = words_to_double(w0, w1);
+        }
+    } else {
+        const int ucount = u_cnt[rb];
+        const int used0 = words ? u_used[rb] : 0;
+        for (int i = tid; i < N; i += nthr) {
+            double uu = NAN;
+            if (words) {
+                const int at = used0 + 2 * i;
+                if (at + 2 <= ucount) uu = words_to_double(words[u_off[rb] + at], words[u_off[rb] + at + 1]);
+            } else if (i < ucount) {
+                uu = uniforms[u_off[rb] + i];
+            }
+            uni[i] = uu;
+        }
+    }
+    __syncthreads();
+    sweep_candidates(st, a0, N, nullptr, cand, ctrl);
+}
+
+// One wave per focal agent k: its social field from the view (X^m for j < k,
+// the start for j > k), the counts, and the decision (agent.py:168-265)
+// into X^{m+1}; the field bits when asked for.
+__global__ void __launch_bounds__(256)
+sweep_field_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, SweepX Xc, SweepX Xn,
+                   const int32_t* __restrict__ cand, const int32_t* __restrict__ ctrl,
+                   uint32_t* __restrict__ bits_out) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t raw[4][kSweepMaxWords];
+    if (ctrl[kSwConverged] | ctrl[kSwSkip]) return;
+    const FGRep& P = reps[rb];
+    const int N = P.n_agents, R = P.n_rays, W = P.words, a0 = P.agent_offset;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + wv;
+    if (k >= N) return;   // uniform over the wave
+    const int gk = a0 + k;
+    const double rk = (double)st.radius[gk];
+    const double pxk = st.x[gk], pyk = st.y[gk];
+    const int pid_k = st.exploited_patch_id[gk];
+    Focal f;
+    {
+        double sn, cs;
+        sincos(st.theta[gk], &sn, &cs);
+        f.px = pxk; f.py = pyk; f.radius = rk;
+        f.cx = pxk + rk;
+        f.cy = pyk + rk;
+        const double ex = pxk + (1.0 + cs) * rk;
+        const double ey = pyk + (1.0 - sn) * rk;
+        const double v1x = ex - f.cx, v1y = ey - f.cy;
+        const double n1 = norm2(v1x, v1y);
+        f.u1x = v1x / n1;
+        f.u1y = v1y / n1;
+    }
+    const FastRec FR = fast_rec_const(R, P.step, P.fov0, P.fov1);
+    const int ncand = ctrl[kSwCand];
+    uint32_t bits[4] = {0u, 0u, 0u, 0u};
+    for (int base = 0; base < ncand; base += 64) {
+        const int q = base + lane;
+        bool live = false;
+        int sx = 0, ex = 0;
+        if (q < ncand) {
+            const int j = cand[q];
+            const int g = a0 + j;
+            const bool moved = j < k;
+            const int over = moved ? Xc.over[j] : st.overriding_mode[g];
+            const int pid = st.exploited_patch_id[g];
+            // agent.py:396-413: a cue is an exploiting agent in range with a
+            // patch other than the focal's own (patchwise exclusion)
+            if (j != k && over == ABM_MODE_EXPLOIT && pid != -1 && !(P.patchwise && pid == pid_k)) {
+                const double ox = moved ? Xc.x[j] : st.x[g];
+                const double oy = moved ? Xc.y[j] : st.y[g];
+                const double rj = (double)st.radius[g];
+                if (norm2((ox + rj) - f.cx, (oy + rj) - f.cy) <= P.vision_range) {
+                    const Rec rc = record_at(ox, oy, kSocial, j, f, R, P.step, P.fov0, P.fov1, FR, false);
+                    live = rc.valid != 0;
+                    sx = rc.sx;
+                    ex = rc.ex;
+                }
+            }
+        }
+        uint64_t m = __ballot(live);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            const int s = __builtin_amdgcn_readlane(sx, l), e = __builtin_amdgcn_readlane(ex, l);
+#pragma unroll
+            for (int qw = 0; qw < 4; ++qw) {
+                const int w = lane + 64 * qw;
+                if (w < W) bits[qw] |= interval_word(w, s, e, R);
+            }
+        }
+    }
+    // counts of the returned field: the kept raw range [ta, tb] and its left half
+    const int nl = R / 2;
+    const int ta = R - 1 - P.keep_hi, tb = R - 1 - P.keep_lo, la = max(R - nl, ta);
+    int tot = 0, left = 0;
+#pragma unroll
+    for (int qw = 0; qw < 4; ++qw) {
+        const int w = lane + 64 * qw;
+        if (w < W) {
+            tot += __popc(bits[qw] & word_range_mask(w, ta, tb));
+            left += __popc(bits[qw] & word_range_mask(w, la, tb));
+        }
+    }
+    const int n_set = wave_sum(tot), n_left = wave_sum(left);
+    if (bits_out) {
+        // soc_v_field[q] = V[R-1-q] on the kept rays
+#pragma unroll
+        for (int qw = 0; qw < 4; ++qw) {
+            const int w = lane + 64 * qw;
+            if (w < W) raw[wv][w] = bits[qw];
+        }
+        fg_wave_sync();
+        uint32_t* dst = bits_out + P.field_offset + (int64_t)k * W;
+        for (int wd = lane; wd < W; wd += 64) {
+            uint32_t mm = 0u;
+            for (int b = 0; b < 32; ++b) {
+                const int q = wd * 32 + b, r = R - 1 - q;
+                if (q >= P.keep_lo && q <= P.keep_hi && ((raw[wv][r >> 5] >> (r & 31)) & 1u)) mm |= 1u << b;
+            }
+            dst[wd] = mm;
+        }
+    }
+    if (lane != 0) return;
+    // ---- the decision of agent k (agent.py:168-265), as in the step kernel
+    double w = st.w[gk], u = st.u[gk];
+    const double vel0 = st.v[gk], cr = st.collected_r[gk], crb = st.collected_r_before[gk];
+    double nov = st.novelty[(int64_t)gk * P.tau];
+    for (int q = 1; q < P.tau; ++q) nov = fmax(nov, st.novelty[(int64_t)gk * P.tau + q]);
+    int over = st.overriding_mode[gk], mode = st.mode[gk];
+    const int env = st.env_status[gk];
+    const double count = (double)n_set, cleft = (double)n_left;
+    const double collected = cr - crb;
+    const double ipriv = P.F_N * nov + P.F_R * collected;
+    const double w_p = w > P.T_w ? w : 0.0;
+    const double u_p = u > P.T_u ? u : 0.0;
+    const double mean = count / (double)R;
+    const double dw = P.Eps_w * mean - P.g_w * (w - P.B_w) - u_p * P.S_uw;
+    const double du = P.Eps_u * ipriv - P.g_u * (u - P.B_u) - w_p * P.S_wu;
+    w = w + dw;
+    u = u + du;
+    if (w > P.w_max) w = P.w_max;
+    if (w < -P.w_max) w = -P.w_max;
+    if (u > P.u_max) u = P.u_max;
+    if (u < -P.u_max) u = -P.u_max;
+    double dvel = 0.0, dth = 0.0;
+    int kind = 1;
+    const bool tw = w > P.T_w, tu = u > P.T_w;   // tr_u compares with T_w
+    auto walk = [&]() {
+        kind = 0;
+        dvel = P.exp_vel_max;
+        over = -1; mode = ABM_MODE_EXPLORE;
+    };
+    auto reloc = [&]() {
+        const double lm = cleft / (double)nl;
+        const double rm = (count - cleft) / (double)(R - nl);
+        dvel = P.exp_vel_max - vel0;
+        dth = (lm - rm) * P.reloc_theta_max;
+        over = -1; mode = ABM_MODE_RELOCATE;
+    };
+    auto exploit = [&]() {
+        over = ABM_MODE_EXPLOIT; mode = ABM_MODE_EXPLOIT;
+        dvel = -vel0 * P.exp_stop_ratio;
+        dth = 0.0;
+    };
+    if (over != ABM_MODE_COLLIDE) {
+        if (!tw && !tu) walk();
+        else if (tw && tu) { if (env == 1) exploit(); else reloc(); }
+        else if (tw && !tu) reloc();
+        else { if (env == 1) exploit(); else walk(); }
+    }
+    Xn.w[k] = w; Xn.u[k] = u; Xn.ip[k] = ipriv;
+    Xn.over[k] = over; Xn.mode[k] = mode; Xn.kind[k] = kind;
+    Xn.dth[k] = dth; Xn.dvel[k] = dvel;
+}
+
+// One block of 1024 threads: the draw indices (a scan of the walk flags in
+// id order), the moves of X^{m+1}, the comparison with X^m and the next
+// candidates.
+__global__ void __launch_bounds__(1024)
+sweep_motion_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, SweepX Xc, SweepX Xn,
+                    const double* __restrict__ uni, int32_t* __restrict__ cand, int32_t* __restrict__ ctrl) {
+#pragma clang fp contract(off)
+    __shared__ int scan[1024];
+    __shared__ int changed;
+    if (ctrl[kSwConverged] | ctrl[kSwSkip]) return;   // uniform over the block
+    const FGRep& P = reps[rb];
+    const int N = P.n_agents, a0 = P.agent_offset, tid = threadIdx.x, nthr = blockDim.x;
+    const int per = (N + nthr - 1) / nthr, lo = tid * per, hi = min(N, lo + per);
+    int mine = 0;
+    for (int k = lo; k < hi; ++k) mine += Xn.kind[k] == 0;
+    scan[tid] = mine;
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    for (int off = 1; off < nthr; off <<= 1) {   // inclusive Hillis-Steele scan
+        const int add = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += add;
+        __syncthreads();
+    }
+    int d = scan[tid] - mine;   // draws before agent lo
+    bool diff = false;
+    auto bits_ne = [](double a, double b) { return __double_as_longlong(a) != __double_as_longlong(b); };
+    for (int k = lo; k < hi; ++k) {
+        const int g = a0 + k;
+        double dth = Xn.dth[k];
+        if (Xn.kind[k] == 0) {
+            const double uu = d < N ? uni[d] : NAN;
+            dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
+            ++d;
+        }
+        Pose p{st.x[g], st.y[g], st.theta[g], st.v[g]};
+        p.theta = prove_orientation(p.theta + dth);
+        double sn, cs;
+        sincos(p.theta, &sn, &cs);
+        p.v = p.v + Xn.dvel[k];
+        // prove_velocity (agent.py:612-620): only in explore mode
+        if (get_mode(Xn.over[k], Xn.w[k], P.T_w) == ABM_MODE_EXPLORE && fabs(p.v) > 1.0) p.v = P.exp_vel_max;
+        p.x = p.x + p.v * cs;
+        p.y = p.y - p.v * sn;
+        p = reflect_walls(p, (double)st.radius[g], P.pad, P.width, P.height);
+        Xn.x[k] = p.x; Xn.y[k] = p.y; Xn.th[k] = p.theta; Xn.v[k] = p.v;
+        diff = diff || bits_ne(p.x, Xc.x[k]) || bits_ne(p.y, Xc.y[k]) || bits_ne(p.theta, Xc.th[k]) ||
+               bits_ne(p.v, Xc.v[k]) || bits_ne(Xn.w[k], Xc.w[k]) || bits_ne(Xn.u[k], Xc.u[k]) ||
+               bits_ne(Xn.ip[k], Xc.ip[k]) || Xn.over[k] != Xc.over[k] || Xn.mode[k] != Xc.mode[k] ||
+               Xn.kind[k] != Xc.kind[k];
+    }
+    if (diff) changed = 1;   // benign race: every writer stores 1
+    __syncthreads();
+    if (tid == nthr - 1) {
+        ctrl[kSwDraws] = scan[nthr - 1];
+        ctrl[kSwIters] += 1;
+    }
+    if (!changed) {
+        if (tid == 0) ctrl[kSwConverged] = 1;
+        return;   // uniform over the block
+    }
+    sweep_candidates(st, a0, N, Xn.over, cand, ctrl);
+}
+
+// Writes the converged sweep into the state (one block of 1024 threads):
+// poses, decision state, collected_r_before, the words used and status, and
+// the device generator advanced by the words drawn.
+__global__ void __launch_bounds__(1024)
+sweep_commit_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, SweepX X,
+                    int32_t* __restrict__ u_used, int32_t* __restrict__ status, int words_or_mt,
+                    abm_mt19937_state* __restrict__ mt, int32_t* __restrict__ ctrl) {
+    __shared__ uint32_t key[kMtN];
+    if (ctrl[kSwSkip]) return;
+    if (!ctrl[kSwConverged]) {
+        if (threadIdx.x == 0) ctrl[kSwBad] = 1;
+        return;
+    }
+    const FGRep& P = reps[rb];
+    const int N = P.n_agents, a0 = P.agent_offset, tid = threadIdx.x, nthr = blockDim.x;
+    for (int i = tid; i < N; i += nthr) {
+        const int g = a0 + i;
+        st.x[g] = X.x[i]; st.y[g] = X.y[i]; st.theta[g] = X.th[i]; st.v[g] = X.v[i];
+        st.w[g] = X.w[i]; st.u[g] = X.u[i]; st.I_priv[g] = X.ip[i];
+        st.collected_r_before[g] = st.collected_r[g];
+        st.mode[g] = X.mode[i]; st.overriding_mode[g] = X.over[i];
+    }
+    const int draws = ctrl[kSwDraws];
+    if (tid == 0 && u_used) u_used[rb] = ctrl[kSwUsed0] + (words_or_mt ? 2 : 1) * draws;
+    if (tid == 0 && status) status[rb] = ABM_WORLD_DONE;
+    if (mt) {
+        // numpy's state after the 2 * draws words: the key twisted v times at
+        // position pos0 + c - 624 v (as the step kernel)
+        const int c = 2 * draws;
+        int v = 0, pos = ctrl[kSwPos0] + c;
+        if (pos > kMtN) {
+            v = (pos - 1) / kMtN;
+            pos -= kMtN * v;
+        }
+        if (v > 0) {
+            for (int i = tid; i < kMtN; i += nthr) key[i] = mt[rb].key[i];
+            __syncthreads();
+            for (int t = 0; t < v; ++t) mt_twist_block(key);
+            for (int i = tid; i < kMtN; i += nthr) mt[rb].key[i] = key[i];
+        }
+        if (tid == 0) mt[rb].pos = pos;
+    }
+}
+
 // ------------------------------------------------ agent-resource interaction
 constexpr int kMaxPatches = 64;     // patches per replica (one 64-bit membership word)
 constexpr int kMaxRetries = 10000;  // add_new_resource_patch's max_retries (sims.py:335)
@@ -1572,6 +1959,71 @@ using namespace abm;
 
 namespace {
 
+constexpr int kForageMaxAgents = 65536;
+
+// Device bytes of the sweep buffers for replicas of up to M agents: two
+// tentative states (nine double and three int arrays each), the uniforms,
+// the candidates and the control words.
+size_t sweep_bytes(int M) {
+    return 2 * (size_t)M * (9 * sizeof(double) + 3 * sizeof(int32_t)) + (size_t)M * sizeof(double) +
+           (size_t)M * sizeof(int32_t) + kSwWords * sizeof(int32_t) + 256;
+}
+
+struct SweepMem {
+    SweepX X[2];
+    double* uni;
+    int32_t* cand;
+    int32_t* ctrl;
+};
+
+SweepMem sweep_carve(void* base, int M) {
+    SweepMem m;
+    char* p = reinterpret_cast<char*>(base);
+    for (int q = 0; q < 2; ++q) {
+        double** d[] = {&m.X[q].x, &m.X[q].y, &m.X[q].th, &m.X[q].v, &m.X[q].w, &m.X[q].u, &m.X[q].ip,
+                        &m.X[q].dth, &m.X[q].dvel};
+        for (double** a : d) { *a = reinterpret_cast<double*>(p); p += sizeof(double) * (size_t)M; }
+    }
+    m.uni = reinterpret_cast<double*>(p); p += sizeof(double) * (size_t)M;
+    for (int q = 0; q < 2; ++q) {
+        int32_t** i[] = {&m.X[q].over, &m.X[q].mode, &m.X[q].kind};
+        for (int32_t** a : i) { *a = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * (size_t)M; }
+    }
+    m.cand = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * (size_t)M;
+    m.ctrl = reinterpret_cast<int32_t*>(p);
+    return m;
+}
+
+// Which replicas the step kernel updates and which are swept: sweeps for
+// those beyond one workgroup's LDS, or (ABM_FORAGE_PATH_SWEEP) for every
+// replica without visual exclusion.  The step kernel's blocks take their
+// replicas largest first (the id-order chain grows with the agents; rays add
+// to each agent's step).
+int plan_paths(abm_forage_batch* bt, int path) {
+    const int n = bt->n_replicas;
+    std::vector<int32_t> small, big;
+    for (int b = 0; b < n; ++b) {
+        const bool sweepable = !bt->exclusion[b] && bt->rep_r[b] <= 32 * kSweepMaxWords;
+        if (bt->too_big[b] || (path == ABM_FORAGE_PATH_SWEEP && sweepable)) big.push_back(b);
+        else small.push_back(b);
+    }
+    auto cost = [&](int b) { return (double)bt->rep_n[b] * (64.0 + bt->rep_r[b] / 32.0); };
+    std::stable_sort(small.begin(), small.end(), [&](int a, int c) { return cost(a) > cost(c); });
+    bt->order_host = small;
+    bt->order_host.insert(bt->order_host.end(), big.begin(), big.end());
+    bt->n_small = (int32_t)small.size();
+    bt->small_max_agents = 0;
+    bt->small_max_rays = 0;
+    for (int b : small) {
+        bt->small_max_agents = std::max(bt->small_max_agents, bt->rep_n[b]);
+        bt->small_max_rays = std::max(bt->small_max_rays, bt->rep_r[b]);
+    }
+    bt->sweep = big;
+    bt->path = path;
+    return check_hip(hipMemcpy(bt->order_dev, bt->order_host.data(), sizeof(int32_t) * n,
+                               hipMemcpyHostToDevice), "upload forage order");
+}
+
 int check_state(const abm_forage_state* st, const char* who) {
     const void* req[] = {st->x, st->y, st->theta, st->v, st->w, st->u, st->I_priv, st->collected_r,
                          st->collected_r_before, st->novelty, st->mode, st->overriding_mode,
@@ -1579,6 +2031,52 @@ int check_state(const abm_forage_state* st, const char* who) {
     for (const void* p : req)
         if (!p) return set_error(ABM_ERR_INVALID, "%s: every state array is required", who);
     return ABM_OK;
+}
+
+// Agent.update of replica rb by sweeps (see sweep_field_kernel): the
+// start, then sweeps in rounds of a few (a sweep after convergence returns
+// at once) until the control words report a fixed point, then the commit.
+// Reads the control words after each round, so the call synchronises the
+// stream.
+int sweep_replica(abm_forage_batch* b, int rb, const abm_forage_state* st, const double* uniforms,
+                  const uint32_t* words, const int64_t* offset, const int32_t* count, int32_t* used,
+                  int32_t* status, uint32_t* field_bits, hipStream_t s, abm_mt19937_state* mt) {
+    const int N = b->rep_n[rb];
+    if (N > b->sweep_max_agents || !b->sweep_mem || !b->sweep_ctrl_host)
+        return set_error(ABM_ERR_UNSUPPORTED, "forage replica %d: no sweep buffers for %d agents", rb, N);
+    const SweepMem m = sweep_carve(b->sweep_mem, b->sweep_max_agents);
+    const FGRep* reps = (const FGRep*)b->reps_dev;
+    hipLaunchKernelGGL(sweep_prep_kernel, dim3(1), dim3(1024), 0, s, reps, rb, *st, uniforms, words, offset,
+                       count, used, status, mt, m.X[0], m.uni, m.cand, m.ctrl);
+    if (int rc = check_hip(hipGetLastError(), "sweep_prep_kernel launch")) return rc;
+    int par = 0, sweeps = 0, round = 3;
+    for (;;) {
+        for (int i = 0; i < round; ++i) {
+            hipLaunchKernelGGL(sweep_field_kernel, dim3((N + 3) / 4), dim3(256), 0, s, reps, rb, *st,
+                               m.X[par], m.X[par ^ 1], m.cand, m.ctrl, field_bits);
+            hipLaunchKernelGGL(sweep_motion_kernel, dim3(1), dim3(1024), 0, s, reps, rb, *st, m.X[par],
+                               m.X[par ^ 1], m.uni, m.cand, m.ctrl);
+            par ^= 1;
+        }
+        if (int rc = check_hip(hipGetLastError(), "sweep kernels launch")) return rc;
+        sweeps += round;
+        if (int rc = check_hip(hipMemcpyAsync(b->sweep_ctrl_host, m.ctrl, sizeof(int32_t) * kSwWords,
+                                              hipMemcpyDeviceToHost, s), "read sweep control"))
+            return rc;
+        if (int rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(sweeps)")) return rc;
+        if (b->sweep_ctrl_host[kSwSkip]) break;
+        if (b->sweep_ctrl_host[kSwConverged]) {
+            b->sweep_steps += 1;
+            b->sweeps_done += b->sweep_ctrl_host[kSwIters];
+            break;
+        }
+        if (sweeps > N + 2)   // agent k is final after k + 1 sweeps: cannot happen
+            return set_error(ABM_ERR_HIP, "forage replica %d: no fixed point after %d sweeps", rb, sweeps);
+        round = 2;
+    }
+    hipLaunchKernelGGL(sweep_commit_kernel, dim3(1), dim3(1024), 0, s, reps, rb, *st, m.X[par], used, status,
+                       (words || mt) ? 1 : 0, mt, m.ctrl);
+    return check_hip(hipGetLastError(), "sweep_commit_kernel launch");
 }
 
 // Agent.update for every replica (all, or those a status array marks ready).
@@ -1590,25 +2088,26 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     // agent's records (wave 1) and the classes of the one after (the other
     // waves): eight waves when some replica has more agents than two waves
     // classify in one pass, else four (two workgroups then share a CU)
-    const int nthr = b->max_agents > 128 ? 512 : 256;
-    const size_t lds = fg_lds_bytes(b->max_agents, b->max_rays);
-    int dev_max = 0;
-    if (int rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                                 b->device), "hipDeviceGetAttribute"))
-        return rc;
-    if (lds + sizeof(FGRep) > (size_t)dev_max)   // the parameters sit in static LDS
-        return set_error(ABM_ERR_UNSUPPORTED, "abm_forage_step_f64: %d agents x %d rays need %zu B of LDS",
-                         b->max_agents, b->max_rays, lds + sizeof(FGRep));
-    if (lds > 65536) {
-        if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                               "hipFuncSetAttribute"))
-            return rc;
+    if (b->n_small > 0) {
+        const int nthr = b->small_max_agents > 128 ? 512 : 256;
+        const size_t lds = fg_lds_bytes(b->small_max_agents, b->small_max_rays);
+        if (lds > 65536) {
+            if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                                   "hipFuncSetAttribute"))
+                return rc;
+        }
+        hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_small), dim3(nthr), lds, (hipStream_t)stream,
+                           (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
+                           field_bits, (const int32_t*)b->order_dev, mt, b->small_max_agents,
+                           b->small_max_rays);
+        if (int rc = check_hip(hipGetLastError(), "forage_step_kernel launch")) return rc;
     }
-    hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_replicas), dim3(nthr), lds, (hipStream_t)stream,
-                       (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
-                       field_bits, (const int32_t*)b->order_dev, mt, b->max_agents, b->max_rays);
-    return check_hip(hipGetLastError(), "forage_step_kernel launch");
+    for (int rb : b->sweep)
+        if (int rc = sweep_replica(b, rb, st, uniforms, words, offset, count, used, status, field_bits,
+                                   (hipStream_t)stream, mt))
+            return rc;
+    return ABM_OK;
 }
 
 }  // namespace
@@ -1624,8 +2123,9 @@ int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forag
     int max_n = 0, max_r = 0;
     for (int b = 0; b < n; ++b) {
         const abm_forage_replica& r = reps[b];
-        if (r.n_agents < 1 || r.n_agents > 4096)
-            return set_error(ABM_ERR_INVALID, "forage replica %d: n_agents=%d out of range [1, 4096]", b, r.n_agents);
+        if (r.n_agents < 1 || r.n_agents > kForageMaxAgents)
+            return set_error(ABM_ERR_INVALID, "forage replica %d: n_agents=%d out of range [1, %d]", b,
+                             r.n_agents, kForageMaxAgents);
         if (r.n_rays < 2 || r.n_rays > (1 << 16))
             return set_error(ABM_ERR_INVALID, "forage replica %d: n_rays=%d out of range", b, r.n_rays);
         if (r.tau < 1 || r.tau > 4096)
@@ -1666,21 +2166,39 @@ int abm_forage_batch_create(const abm_forage_replica* reps, int32_t n, abm_forag
     bt->n_replicas = n; bt->n_agents_total = (int32_t)agents; bt->field_words = words;
     bt->max_agents = max_n; bt->max_rays = max_r;
     int rc = check_hip(hipGetDevice(&bt->device), "hipGetDevice");
+    int dev_max = 0;
+    if (!rc) rc = check_hip(hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                  bt->device), "hipDeviceGetAttribute");
+    // a replica beyond one workgroup's LDS is updated by sweeps, which do
+    // not implement visual exclusion and hold at most 8192 rays
+    bt->too_big.assign(n, 0);
+    bt->exclusion.assign(n, 0);
+    for (int b = 0; !rc && b < n; ++b) {
+        const size_t need = fg_lds_bytes(host[b].n_agents, host[b].n_rays) + sizeof(FGRep);
+        bt->too_big[b] = need > (size_t)dev_max;
+        bt->exclusion[b] = host[b].visual_exclusion != 0;
+        if (bt->too_big[b] && (host[b].visual_exclusion || host[b].n_rays > 32 * kSweepMaxWords))
+            rc = set_error(ABM_ERR_UNSUPPORTED, "forage replica %d: %d agents x %d rays need %zu B of LDS "
+                           "(one workgroup has %d B); replicas that large are updated by sweeps, which "
+                           "support neither visual exclusion nor more than %d rays", b, host[b].n_agents,
+                           host[b].n_rays, need, dev_max, 32 * kSweepMaxWords);
+        if (!host[b].visual_exclusion && host[b].n_rays <= 32 * kSweepMaxWords)
+            bt->sweep_max_agents = std::max(bt->sweep_max_agents, host[b].n_agents);
+    }
+    bt->order_host.resize(n);
+    bt->rep_n.resize(n);
+    bt->rep_r.resize(n);
+    for (int b = 0; b < n; ++b) { bt->rep_n[b] = host[b].n_agents; bt->rep_r[b] = host[b].n_rays; }
     if (!rc) rc = check_hip(hipMalloc(&bt->reps_dev, sizeof(FGRep) * n), "hipMalloc(forage replicas)");
     if (!rc) rc = check_hip(hipMemcpy(bt->reps_dev, host.data(), sizeof(FGRep) * n, hipMemcpyHostToDevice),
                            "upload forage replicas");
-    // update order of the blocks: largest replicas first (the id-order chain
-    // of a replica grows with its agents; rays add to each agent's step)
-    std::vector<int32_t> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-        const double ca = (double)host[a].n_agents * (64.0 + host[a].n_rays / 32.0);
-        const double cc = (double)host[c].n_agents * (64.0 + host[c].n_rays / 32.0);
-        return ca > cc;
-    });
     if (!rc) rc = check_hip(hipMalloc(&bt->order_dev, sizeof(int32_t) * n), "hipMalloc(forage order)");
-    if (!rc) rc = check_hip(hipMemcpy(bt->order_dev, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice),
-                           "upload forage order");
+    if (!rc && bt->sweep_max_agents > 0)
+        rc = check_hip(hipMalloc(&bt->sweep_mem, sweep_bytes(bt->sweep_max_agents)), "hipMalloc(sweeps)");
+    if (!rc && bt->sweep_max_agents > 0)
+        rc = check_hip(hipHostMalloc((void**)&bt->sweep_ctrl_host, sizeof(int32_t) * kSwWords),
+                       "hipHostMalloc(sweep control)");
+    if (!rc) rc = plan_paths(bt, ABM_FORAGE_PATH_AUTO);
     if (rc) { abm_forage_batch_destroy(bt); return rc; }
     bt->agent_offset_host = new int32_t[n + 1];
     bt->field_offset_host = new int64_t[n + 1];
@@ -1698,6 +2216,8 @@ int abm_forage_batch_destroy(abm_forage_batch* b) {
     if (!b) return ABM_OK;
     if (b->reps_dev) (void)hipFree(b->reps_dev);
     if (b->order_dev) (void)hipFree(b->order_dev);
+    if (b->sweep_mem) (void)hipFree(b->sweep_mem);
+    if (b->sweep_ctrl_host) (void)hipHostFree(b->sweep_ctrl_host);
     delete[] b->agent_offset_host;
     delete[] b->field_offset_host;
     delete b;
@@ -1718,6 +2238,20 @@ int abm_forage_batch_get_offsets(const abm_forage_batch* b, int32_t* agent_offse
     return ABM_OK;
 }
 
+
+int abm_forage_sweep_counts(const abm_forage_batch* b, int64_t* updates, int64_t* sweeps) {
+    if (!b || !updates || !sweeps) return set_error(ABM_ERR_INVALID, "abm_forage_sweep_counts: NULL argument");
+    *updates = b->sweep_steps;
+    *sweeps = b->sweeps_done;
+    return ABM_OK;
+}
+
+int abm_forage_set_path(abm_forage_batch* b, int32_t path) {
+    if (!b) return set_error(ABM_ERR_INVALID, "abm_forage_set_path: NULL batch");
+    if (path != ABM_FORAGE_PATH_AUTO && path != ABM_FORAGE_PATH_SWEEP)
+        return set_error(ABM_ERR_INVALID, "abm_forage_set_path: unknown path %d", path);
+    return plan_paths(b, path);
+}
 
 int abm_forage_step_f64(abm_forage_batch* b, const abm_forage_state* st, const double* uniforms,
                         const int64_t* uniform_offset, const int32_t* uniform_count,

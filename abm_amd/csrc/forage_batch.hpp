@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/abm_amd.h"
 
 struct abm_forage_batch {
@@ -17,6 +19,20 @@ struct abm_forage_batch {
     void* order_dev = nullptr;        // replicas by descending cost: the blocks' update order
     int32_t* agent_offset_host = nullptr;
     int64_t* field_offset_host = nullptr;
+    // Replicas the step kernel updates (the first n_small of order_dev) and
+    // the ones it cannot hold in one workgroup's LDS, updated by sweeps.
+    int32_t n_small = 0;
+    int32_t small_max_agents = 0, small_max_rays = 0;
+    int32_t path = 0;                  // ABM_FORAGE_PATH_*
+    std::vector<int32_t> rep_n, rep_r; // agents and rays of every replica
+    std::vector<int32_t> order_host;   // every replica, step-kernel ones first by cost
+    std::vector<uint8_t> too_big;      // [b]: beyond one workgroup's LDS
+    std::vector<uint8_t> exclusion;    // [b]: visual exclusion
+    std::vector<int32_t> sweep;        // replicas updated by sweeps
+    int32_t sweep_max_agents = 0;      // the sweep buffers' capacity
+    void* sweep_mem = nullptr;         // two tentative states, uniforms, candidates, control
+    int32_t* sweep_ctrl_host = nullptr;   // pinned copy of the control words
+    int64_t sweep_steps = 0, sweeps_done = 0;   // replica updates by sweeps, sweeps they took
 };
 
 // The resource patches' per-replica constants and pending-placement scratch

@@ -165,6 +165,21 @@ class ForageSwarm:
     def replica_slice(self, b):
         return slice(int(self.agent_offset[b]), int(self.agent_offset[b + 1]))
 
+    def set_path(self, path: str = "auto"):
+        """How replicas are updated: "auto" (the step kernel for replicas that
+        fit one workgroup's LDS, sweeps for larger ones) or "sweep" (sweeps for
+        every replica without visual exclusion; the same results, used to
+        compare the two).  Sweeps read a few control words per round, so a
+        step with swept replicas synchronises the stream."""
+        code = {"auto": L.FORAGE_PATH_AUTO, "sweep": L.FORAGE_PATH_SWEEP}[path]
+        L.check(self.lib.abm_forage_set_path(self.handle, code))
+
+    def sweep_counts(self):
+        """(replica updates made by sweeps, sweeps they took) since creation."""
+        u, s = C.c_int64(), C.c_int64()
+        L.check(self.lib.abm_forage_sweep_counts(self.handle, C.byref(u), C.byref(s)))
+        return u.value, s.value
+
     # ---------------------------------------------------------------- step
     def _state_addrs(self):
         """Addresses of the replicas' MT19937 states (numpy's mt19937_state,

@@ -342,7 +342,9 @@ int abm_forage_batch_get_offsets(const abm_forage_batch* batch, int32_t* agent_o
  * workgroup per replica (256 threads, or 512 when a replica has more than
  * 128 agents) holds the replica in LDS: about 172 bytes per agent plus one
  * byte per ray, so up to about 920 agents per replica at R = 1200 on gfx950
- * (160 KB); larger replicas get ABM_ERR_UNSUPPORTED. */
+ * (160 KB).  Larger replicas (up to 65536 agents) are updated by sweeps (see
+ * abm_forage_set_path); with visual exclusion they get ABM_ERR_UNSUPPORTED
+ * from abm_forage_batch_create. */
 /* Arguments:
  * uniforms: per replica, a device array of the next numpy legacy-MT19937
  *   doubles (random_sample) of the global stream, uniform_offset[b] its start
@@ -356,6 +358,22 @@ int abm_forage_step_f64(abm_forage_batch* batch, const abm_forage_state* state,
                         const double* uniforms, const int64_t* uniform_offset,
                         const int32_t* uniform_count, int32_t* uniforms_used,
                         uint32_t* field_bits, void* stream);
+
+/* Update paths of abm_forage_step_f64 and abm_forage_world_run* (no
+ * counterpart in the reference, which has one Python loop).  A replica that
+ * fits one workgroup's LDS is updated by the step kernel; a larger one
+ * (visual exclusion off, at most 8192 rays) by sweeps: every agent's update
+ * recomputed at once from the latest tentative states of the agents before
+ * it, repeated until nothing changes, which is the reference's id-order
+ * result exactly.  A call with such a replica reads a few control words per
+ * round of sweeps, so it synchronises the stream.  ABM_FORAGE_PATH_SWEEP
+ * sweeps every replica without visual exclusion (same results). */
+#define ABM_FORAGE_PATH_AUTO  0
+#define ABM_FORAGE_PATH_SWEEP 1
+int abm_forage_set_path(abm_forage_batch* batch, int32_t path);
+/* Replica updates made by sweeps since the batch was created, and the sweeps
+ * they took (the last one of each confirms the fixed point). */
+int abm_forage_sweep_counts(const abm_forage_batch* batch, int64_t* updates, int64_t* sweeps);
 
 /* Agent.projection_field for one focal agent (abm/agent/agent.py:457-597).
  * Obstacles are the social cues, followed by n_nonsocial non-social

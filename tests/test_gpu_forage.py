@@ -194,22 +194,25 @@ def test_eight_wave_pipeline_matches_oracle(gpu, vis_ex):
 
 def test_replica_size_limit(gpu):
     # a replica lives in one workgroup's LDS (172 bytes per agent plus a byte
-    # per ray, 160 KB on gfx950): 920 agents at R = 1200 run, 940 get
-    # ABM_ERR_UNSUPPORTED naming the LDS they need
+    # per ray, 160 KB on gfx950): 920 agents at R = 1200 fit; 940 do not and
+    # are updated by sweeps instead (test_gpu_sweep.py), which refuse visual
+    # exclusion
     from abm_amd import _lib as L
     from abm_amd.forage import ForageConfig, ForageSwarm
     W = 3000.0
-    for N, fits in [(920, True), (940, False)]:
+    for N, vis_ex in [(920, True), (940, False), (940, True)]:
         rng = np.random.RandomState(3)
-        sw = ForageSwarm([ForageConfig(n_agents=N, n_rays=1200, width=W, height=W, vision_range=2000.0)],
-                         rng.uniform(30, 30 + W - 20, N), rng.uniform(30, 30 + W - 20, N),
-                         rng.uniform(0, 2 * PI, N), radius=10, rngs=[np.random.RandomState(1)])
-        if fits:
-            sw.step()
-            assert np.isfinite(sw.get_state("x")["x"]).all()
-        else:
+        args = ([ForageConfig(n_agents=N, n_rays=1200, width=W, height=W, vision_range=200.0,
+                              visual_exclusion=vis_ex)],
+                rng.uniform(30, 30 + W - 20, N), rng.uniform(30, 30 + W - 20, N),
+                rng.uniform(0, 2 * PI, N))
+        if N == 940 and vis_ex:
             with pytest.raises(L.AbmUnsupported, match="LDS"):
-                sw.step()
+                ForageSwarm(*args, radius=10, rngs=[np.random.RandomState(1)])
+            continue
+        sw = ForageSwarm(*args, radius=10, rngs=[np.random.RandomState(1)])
+        sw.step()
+        assert np.isfinite(sw.get_state("x")["x"]).all()
 
 
 @pytest.mark.parametrize("fov_ratio", [1.0, 0.5])
