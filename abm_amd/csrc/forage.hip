@@ -1506,6 +1506,7 @@ sweep_commit_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
 
 // ------------------------------------------------ agent-resource interaction
 constexpr int kMaxPatches = 64;     // patches per replica (one 64-bit membership word)
+constexpr int kMaxWorldChunks = 1024;   // 64-agent words of a replica's membership (65536 agents)
 constexpr int kMaxRetries = 10000;  // add_new_resource_patch's max_retries (sims.py:335)
 
 // Pending patch placements of a replica, kept between a phase that ran out of
@@ -1695,8 +1696,8 @@ __device__ int place_patches(const WRep& W, abm_patch_state ps, int b, int kind,
 // then the placement of regenerated patches; FILL: create_resources; RESUME:
 // the placements a previous phase left pending.  Membership is computed for
 // every (patch, agent) pair in parallel from the poses at the start of the
-// step; the loop over patches and their agents, whose depletion, notices and
-// kills depend on the order, runs in id order in one thread.
+// step; the loop over patches and their agents runs per agent in parallel,
+// with only each patch's depletion chain in id order (one thread per patch).
 __global__ void __launch_bounds__(256)
 forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wreps, int phase,
                     abm_forage_state st, int32_t* __restrict__ pool_success, abm_patch_state ps,
@@ -1711,8 +1712,7 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
     __shared__ uint32_t mkey[kMtN];
     __shared__ int s_kind, s_pending, s_pos;
     const int b = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
-    // the replica's parameters in LDS, read where used (held in scalar
-    // registers over thread 0's serial walk they spill)
+    // the replica's parameters in LDS, read where used
     __shared__ FGRep sP;
     __shared__ WRep sW;
     static_assert(sizeof(WRep) % 8 == 0 && sizeof(WRep) / 8 <= 64, "WRep copy");
@@ -1743,8 +1743,7 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
     }
     if (phase == ABM_WORLD_STEP) {
         const int np_ = n_live;
-        // the live patches' geometry, quality and units staged in LDS, so
-        // thread 0's walk over the groups below waits on no global load per patch
+        // the live patches' geometry, quality and units staged in LDS
         __shared__ double s_pr[kMaxPatches], s_px[kMaxPatches], s_py[kMaxPatches];
         __shared__ double s_pq[kMaxPatches], s_pl[kMaxPatches];
         __shared__ int32_t s_pid[kMaxPatches];
@@ -1772,80 +1771,194 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
             if ((tid & 63) == 0) member[p * C + (j >> 6)] = m;
         }
         __syncthreads();
-        // ---- the patches' agent groups in order (sims.py:803-845)
-        if (tid == 0) {
-            int n_kill = 0;
+        // ---- the patches' agent groups (sims.py:803-845).  The reference
+        // walks the patches in Group order and their agents in id order; an
+        // agent's centre lies in at most one patch (patches never overlap),
+        // so only the depletion chain of each patch is sequential (patches a
+        // caller loaded that overlap take the serial walk).  Phase A
+        // (every thread): per agent on a patch, the bias towards the patch
+        // centre and its mode and pooling flags, read before any notify of
+        // the walk (a notify never changes them; a later notify of the same
+        // agent only follows its own pooling test).  Phase B (one thread per
+        // patch): the exploiting agents deplete the patch in id order until
+        // its last unit; the agent that took it is the destroyer.  Phase C
+        // (every thread): each agent's notifies in the walk's order: a
+        // pooled agent up to the destroyer hears of the patch (and teleports),
+        // then every agent of a destroyed patch hears -1 from the
+        // destroyer's notify-all, and agents after the destroyer hear -1
+        // once more in their own turn.
+        __shared__ uint64_t s_on[kMaxWorldChunks], s_expl[kMaxWorldChunks], s_pool[kMaxWorldChunks];
+        __shared__ int32_t s_destroyer[kMaxPatches];
+        __shared__ int s_multi;
+        if (tid == 0) s_multi = 0;
+        __syncthreads();
+        for (int c = tid; c < C; c += nthr) {
+            uint64_t on = 0, dup = 0;
             for (int p = 0; p < np_; ++p) {
-                const int64_t s = po + p;
-                const double pr = s_pr[p];
-                const double pcx = s_px[p] + pr, pcy = s_py[p] + pr;   // resc.center
-                const int32_t pid = s_pid[p];
-                bool destroy = false;
-                for (int c = 0; c < C; ++c) {
-                    uint64_t m = member[p * C + c];
-                    while (m) {
-                        const int g = a0 + c * 64 + __builtin_ctzll(m);
-                        m &= m - 1;
-                        const double ra = (double)st.radius[g];
-                        // bias_agent_towards_res_center (sims.py:544-552)
-                        const double x1 = st.x[g] + ra, y1 = st.y[g] + ra;
-                        const double th = st.theta[g];
-                        const double cl = py_mod_2pi(atan2(pcy - y1, pcx - x1) + th);
-                        st.theta[g] = th + (cl - kPi) * 0.02;
-                        if (destroy) {
-                            notify(st, pool_success, P.tau, g, -1, -1);
-                            continue;
-                        }
-                        const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
-                        if (pooled(mode, pool_success[g], W.pooling_time)) {
-                            notify(st, pool_success, P.tau, g, 1, pid);
-                            if (W.teleport) {
-                                st.x[g] = (s_px[p] + pr) - ra;
-                                st.y[g] = (s_py[p] + pr) - ra;
-                            }
-                        }
-                        if (mode == ABM_MODE_EXPLOIT) {
-                            // Rescource.deplete (rescource.py:118-133)
-                            double units = W.consumption;
-                            const double q = s_pq[p];
-                            if (units > q) units = q;
-                            double left = s_pl[p], dep;
-                            if (left >= units) { left = left - units; dep = units; }
-                            else { dep = left; left = 0.0; }
-                            s_pl[p] = left;
-                            ps.left[s] = left;
-                            destroy = !(left > 0.0);
-                            const double cr = st.collected_r[g];
-                            st.collected_r_before[g] = cr;
-                            st.collected_r[g] = cr + dep;
+                dup |= on & member[p * C + c];
+                on |= member[p * C + c];
+            }
+            s_on[c] = on;
+            if (dup) s_multi = 1;
+        }
+        for (int p = tid; p < np_; p += nthr) s_destroyer[p] = N;
+        __syncthreads();
+        if (s_multi) {
+            // patches loaded by the caller that overlap: the walk in the
+            // reference's order, in one thread
+            if (tid == 0) {
+                int n_kill = 0;
+                for (int p = 0; p < np_; ++p) {
+                    const int64_t s = po + p;
+                    const double pr = s_pr[p];
+                    const double pcx = s_px[p] + pr, pcy = s_py[p] + pr;   // resc.center
+                    const int32_t pid = s_pid[p];
+                    bool destroy = false;
+                    for (int c = 0; c < C; ++c) {
+                        uint64_t m = member[p * C + c];
+                        while (m) {
+                            const int g = a0 + c * 64 + __builtin_ctzll(m);
+                            m &= m - 1;
+                            const double ra = (double)st.radius[g];
+                            // bias_agent_towards_res_center (sims.py:544-552)
+                            const double x1 = st.x[g] + ra, y1 = st.y[g] + ra;
+                            const double th = st.theta[g];
+                            const double cl = py_mod_2pi(atan2(pcy - y1, pcx - x1) + th);
+                            st.theta[g] = th + (cl - kPi) * 0.02;
                             if (destroy) {
-                                // every agent of the group hears of it
-                                for (int c2 = 0; c2 < C; ++c2) {
-                                    uint64_t m2 = member[p * C + c2];
-                                    while (m2) {
-                                        const int g2 = a0 + c2 * 64 + __builtin_ctzll(m2);
-                                        m2 &= m2 - 1;
-                                        notify(st, pool_success, P.tau, g2, -1, -1);
+                                notify(st, pool_success, P.tau, g, -1, -1);
+                                continue;
+                            }
+                            const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
+                            if (pooled(mode, pool_success[g], W.pooling_time)) {
+                                notify(st, pool_success, P.tau, g, 1, pid);
+                                if (W.teleport) {
+                                    st.x[g] = (s_px[p] + pr) - ra;
+                                    st.y[g] = (s_py[p] + pr) - ra;
+                                }
+                            }
+                            if (mode == ABM_MODE_EXPLOIT) {
+                                // Rescource.deplete (rescource.py:118-133)
+                                double units = W.consumption;
+                                const double q = s_pq[p];
+                                if (units > q) units = q;
+                                double left = s_pl[p], dep;
+                                if (left >= units) { left = left - units; dep = units; }
+                                else { dep = left; left = 0.0; }
+                                s_pl[p] = left;
+                                ps.left[s] = left;
+                                destroy = !(left > 0.0);
+                                const double cr = st.collected_r[g];
+                                st.collected_r_before[g] = cr;
+                                st.collected_r[g] = cr + dep;
+                                if (destroy) {
+                                    // every agent of the group hears of it
+                                    for (int c2 = 0; c2 < C; ++c2) {
+                                        uint64_t m2 = member[p * C + c2];
+                                        while (m2) {
+                                            const int g2 = a0 + c2 * 64 + __builtin_ctzll(m2);
+                                            m2 &= m2 - 1;
+                                            notify(st, pool_success, P.tau, g2, -1, -1);
+                                        }
                                     }
                                 }
                             }
                         }
                     }
+                    if (destroy) {
+                        kill_rank[s] = n_kill;
+                        pend_id[po + n_kill] = pid;
+                        ++n_kill;
+                    }
                 }
-                if (destroy) {
-                    kill_rank[s] = n_kill;
-                    pend_id[po + n_kill] = pid;
+                kind = n_kill ? kPendKill : kPendNone;
+                n_pending = n_kill;
+            }
+        } else {
+            for (int base = tid & ~63; base < C * 64; base += nthr) {
+                const int j = base + (tid & 63);
+                bool ex = false, pl = false;
+                if (j < N && ((s_on[j >> 6] >> (j & 63)) & 1ull)) {
+                    int p = 0;
+                    while (!((member[p * C + (j >> 6)] >> (j & 63)) & 1ull)) ++p;
+                    const int g = a0 + j;
+                    const double pr = s_pr[p];
+                    const double pcx = s_px[p] + pr, pcy = s_py[p] + pr;   // resc.center
+                    const double ra = (double)st.radius[g];
+                    // bias_agent_towards_res_center (sims.py:544-552)
+                    const double x1 = st.x[g] + ra, y1 = st.y[g] + ra;
+                    const double th = st.theta[g];
+                    const double cl = py_mod_2pi(atan2(pcy - y1, pcx - x1) + th);
+                    st.theta[g] = th + (cl - kPi) * 0.02;
+                    const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
+                    ex = mode == ABM_MODE_EXPLOIT;
+                    pl = pooled(mode, pool_success[g], W.pooling_time);
+                }
+                const uint64_t me = __ballot(ex), mp = __ballot(pl);
+                if ((tid & 63) == 0 && base < C * 64) { s_expl[base >> 6] = me; s_pool[base >> 6] = mp; }
+            }
+            __syncthreads();
+            for (int p = tid; p < np_; p += nthr) {
+                // Rescource.deplete (rescource.py:118-133) by the exploiting
+                // agents of patch p in id order
+                const int64_t s = po + p;
+                double units = W.consumption;
+                const double q = s_pq[p];
+                if (units > q) units = q;
+                double left = s_pl[p];
+                bool destroy = false;
+                for (int c = 0; c < C && !destroy; ++c) {
+                    uint64_t m = member[p * C + c] & s_expl[c];
+                    while (m && !destroy) {
+                        const int j = c * 64 + __builtin_ctzll(m);
+                        m &= m - 1;
+                        const int g = a0 + j;
+                        double dep;
+                        if (left >= units) { left = left - units; dep = units; }
+                        else { dep = left; left = 0.0; }
+                        destroy = !(left > 0.0);
+                        const double cr = st.collected_r[g];
+                        st.collected_r_before[g] = cr;
+                        st.collected_r[g] = cr + dep;
+                        if (destroy) s_destroyer[p] = j;
+                    }
+                }
+                s_pl[p] = left;
+                ps.left[s] = left;
+            }
+            __syncthreads();
+            for (int j = tid; j < N; j += nthr) {
+                if (!((s_on[j >> 6] >> (j & 63)) & 1ull)) continue;
+                int p = 0;
+                while (!((member[p * C + (j >> 6)] >> (j & 63)) & 1ull)) ++p;
+                const int g = a0 + j;
+                const int d = s_destroyer[p];
+                if (j <= d && ((s_pool[j >> 6] >> (j & 63)) & 1ull)) {
+                    notify(st, pool_success, P.tau, g, 1, s_pid[p]);
+                    if (W.teleport) {
+                        const double pr = s_pr[p], ra = (double)st.radius[g];
+                        st.x[g] = (s_px[p] + pr) - ra;
+                        st.y[g] = (s_py[p] + pr) - ra;
+                    }
+                }
+                if (d < N) notify(st, pool_success, P.tau, g, -1, -1);   // the destroyer's notify-all
+                if (j > d) notify(st, pool_success, P.tau, g, -1, -1);   // its own turn after it
+            }
+            if (tid == 0) {
+                int n_kill = 0;
+                for (int p = 0; p < np_; ++p) {
+                    if (s_destroyer[p] >= N) continue;
+                    kill_rank[po + p] = n_kill;
+                    pend_id[po + n_kill] = s_pid[p];
                     ++n_kill;
                 }
+                kind = n_kill ? kPendKill : kPendNone;
+                n_pending = n_kill;
             }
-            kind = n_kill ? kPendKill : kPendNone;
-            n_pending = n_kill;
         }
         // ---- agents on no patch (sims.py:847-855)
         for (int j = tid; j < N; j += nthr) {
-            bool on = false;
-            for (int p = 0; p < np_; ++p) on |= ((member[p * C + (j >> 6)] >> (j & 63)) & 1ull) != 0;
-            if (on) continue;
+            if ((s_on[j >> 6] >> (j & 63)) & 1ull) continue;
             const int g = a0 + j;
             const int mode = get_mode(st.overriding_mode[g], st.w[g], P.T_w);
             if (pooled(mode, pool_success[g], W.pooling_time) || mode == ABM_MODE_EXPLOIT)
