@@ -24,6 +24,7 @@ ORDER_SYNCHRONOUS = 1
 WANT_DIAG = 2
 COUNT_RECHECKS = 4
 VERIFY_FAST_PATH = 8
+KAHAN_SUMS = 16
 
 MODES = ["explore", "exploit", "relocate", "collide", "pool"]   # ABM_MODE_* order
 
@@ -142,6 +143,7 @@ _SIGS = {
     "abm_vf_step_f32": (C.c_int, [_P, C.POINTER(VFStateF32), _P, _P, C.c_uint32, _P]),
     "abm_vf_f32_fallback_count": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                             C.POINTER(C.c_int64), _P]),
+    "abm_vf_f64_near_ties": (C.c_int, [_P, _P, _P]),
     "abm_vf_set_launch": (C.c_int, [_P, _I32]),
     "abm_vf_set_grouping": (C.c_int, [_P, _I32]),
     "abm_vf_autotune": (C.c_int, [_P, C.POINTER(VFStateF32), _I32, C.POINTER(_I32),

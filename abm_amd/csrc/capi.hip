@@ -326,6 +326,19 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
         const double wc = (double)host[c].n_agents * host[c].n_agents * (64.0 + host[c].n_rays / 8.0);
         return wa > wc;
     });
+    // the large size class first (still by descending work)
+    std::stable_partition(order.begin(), order.end(),
+                          [&](int b) { return host[b].n_agents > kBigClass; });
+    for (int b = 0; b < n; ++b) {
+        if (host[b].n_agents > kBigClass) {
+            ++bt->n_big_class;
+            bt->big_max_agents = std::max(bt->big_max_agents, host[b].n_agents);
+            bt->big_max_rays = std::max(bt->big_max_rays, host[b].n_rays);
+        } else {
+            bt->small_max_agents = std::max(bt->small_max_agents, host[b].n_agents);
+            bt->small_max_rays = std::max(bt->small_max_rays, host[b].n_rays);
+        }
+    }
     if (!rc) rc = check_hip(hipMalloc(&bt->reps_dev, sizeof(VFRep) * n), "hipMalloc(replicas)");
     if (!rc) rc = check_hip(hipMalloc(&bt->tab_cos, sizeof(double) * 3 * ray_total), "hipMalloc(tables)");
     if (!rc) rc = check_hip(hipMalloc(&bt->tab32, sizeof(RayTab32) * ray_total), "hipMalloc(tab32)");
@@ -449,10 +462,20 @@ int abm_vf_f32_fallback_count(abm_vf_batch* b, int64_t* pairs_total, int64_t* pa
     return ABM_OK;
 }
 
+int abm_vf_f64_near_ties(abm_vf_batch* b, int64_t* near_ties, void* stream) {
+    if (!b || !near_ties) return set_error(ABM_ERR_INVALID, "abm_vf_f64_near_ties: NULL argument");
+    unsigned long long h[4] = {0, 0, 0, 0};
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = check_hip(hipMemcpyAsync(h, b->counters, sizeof(h), hipMemcpyDeviceToHost, s), "copy counters")) return rc;
+    if (int rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return rc;
+    *near_ties = (int64_t)h[3];
+    return ABM_OK;
+}
+
 int abm_vf_set_launch(abm_vf_batch* b, int32_t waves) {
     if (!b) return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: NULL batch");
-    if (waves != 0 && waves != 1 && waves != 2 && waves != 4)
-        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2 or 4");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 16)
+        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2, 4 or 16");
     b->waves_per_group = waves;
     b->group_g = waves > 0 ? -1 : 0;
     return ABM_OK;
@@ -482,13 +505,17 @@ int abm_vf_autotune(abm_vf_batch* b, const abm_vf_state_f32* st, int32_t trial_s
     if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) { (void)hipEventDestroy(e0); return rc; }
     // candidates: waves per replica of the per-replica kernel (> 0), or minus the
     // replicas per wave of the grouped kernel
+    // (16: the 16-wave kernel for every replica; 0: the size classes, the
+    // per-replica kernel up to kBigClass agents and the 16-wave kernel above)
     std::vector<int> cand{1, 2, 4};
+    if (b->max_agents <= 2048) cand.push_back(16);
+    if (b->n_big_class > 0 && b->n_big_class < b->n_replicas) cand.push_back(0);
     if (b->group_ok) { cand.push_back(-1); cand.push_back(-2); cand.push_back(-4); }
     int best = 1;
     float best_ms = 1e30f;
     int rc = ABM_OK;
     for (size_t c = 0; c < cand.size() && !rc; ++c) {
-        if (cand[c] > 0) { b->waves_per_group = cand[c]; b->group_g = -1; }
+        if (cand[c] >= 0) { b->waves_per_group = cand[c]; b->group_g = -1; }
         else { b->waves_per_group = 0; b->group_g = -cand[c]; }
         rc = launch_vf_step_f32(b, st, nullptr, nullptr, 0, s);  // warm-up
         if (!rc) rc = check_hip(hipEventRecord(e0, s), "hipEventRecord");
@@ -502,7 +529,7 @@ int abm_vf_autotune(abm_vf_batch* b, const abm_vf_state_f32* st, int32_t trial_s
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    if (best > 0) { b->waves_per_group = best; b->group_g = -1; }
+    if (best >= 0) { b->waves_per_group = best; b->group_g = -1; }
     else { b->waves_per_group = 0; b->group_g = -best; }
     if (chosen) *chosen = best;
     if (chosen_ms) *chosen_ms = best_ms / (float)trial_steps;

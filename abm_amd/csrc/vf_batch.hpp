@@ -57,6 +57,11 @@ struct GroupPlan {
     int32_t tab_entries = 0;     // float2 entries per table
 };
 
+// Replicas above this many agents form the large size class of a ragged
+// batch: the 16-wave (one workgroup per replica) kernel updates them, the
+// per-replica kernel the rest.
+constexpr int kBigClass = 128;
+
 // Largest replica the grouped kernel takes, and the largest R.
 constexpr int kGroupMaxAgents = 256;
 constexpr int kGroupMaxRays = 1024;
@@ -78,14 +83,18 @@ struct abm_vf_batch {
     int device = 0;
     int dev_max_lds = 0;            // device limits, read once at creation
     int n_cu = 0;
-    int occ_f32[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [sync][nw 1,2,4]: groups per CU
+    int occ_f32[2][2][3] = {};      // [size classes][sync][nw 1,2,4]: groups per CU
     abm::VFRep* reps_dev = nullptr;
     double* tab_cos = nullptr;      // float64 tables: cos(PHI), sin(PHI), diff(PHI)
     double* tab_sin = nullptr;
     double* tab_dphi = nullptr;
     abm::RayTab32* tab32 = nullptr;
-    unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks
-    int32_t* order_dev = nullptr;   // replica launch order (largest work first)
+    unsigned long long* counters = nullptr;   // [0] pairs, [1] float64 re-checks, [2] disagreements, [3] float64 near ties
+    int32_t* order_dev = nullptr;   // replica launch order: the large size class first, each
+                                    // class by descending work
+    int32_t n_big_class = 0;        // replicas above kBigClass agents (first in order_dev)
+    int32_t small_max_agents = 0, small_max_rays = 0;   // of the other replicas
+    int32_t big_max_agents = 0, big_max_rays = 0;       // of the large class
     float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
     float4* grid_sorted = nullptr;  // wide step: the replica's agents sorted by grid cell
     int32_t* grid_start = nullptr;  // wide step: first sorted agent of each cell (+ end)

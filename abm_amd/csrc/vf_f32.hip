@@ -732,6 +732,272 @@ int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, flo
     return ABM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Reference (id) order for replicas beyond kWideThreshold agents: one
+// workgroup of 1024 threads per replica, agents in id order as in the
+// per-replica kernel, with every agent's pose (x, y, radius, theta) in LDS,
+// 16 bytes each (up to about 8,900 agents in 160 KB).  Velocities and the
+// ray table stay in global memory (each thread keeps its rays' table entries
+// in registers for the whole step).  Per focal agent k: a cheap float32 pass
+// over all others drops the pairs beyond the exact visibility cut-off (as
+// the wide kernel) and queues the rest in LDS (16-bit indices); the queued
+// pairs take the float32 decision (float64 inside the guard bands) and are
+// ORed into the field; each thread sums its rays, the waves' sums meet in
+// LDS, wave 0 moves agent k.
+constexpr int kBigThreads = 1024;
+constexpr int kBigWaves = kBigThreads / 64;
+
+// Queue segment of one wave: wave w owns the objects j with j mod 1024 in
+// [64 w, 64 w + 64), at most 64 per 1024 agents.
+__host__ __device__ inline int big_segment(int maxN) { return 64 * ((maxN + kBigThreads - 1) / kBigThreads); }
+
+// The queues of 16-bit indices, padded to 16 bytes (what follows is int32).
+__host__ __device__ inline size_t big_queue_bytes(int maxN) {
+    return ((size_t)kBigWaves * big_segment(maxN) * sizeof(uint16_t) + 15) & ~(size_t)15;
+}
+
+size_t lds_bytes_big(int maxN, int maxR) {
+    const int maxW = (maxR + 31) >> 5;
+    size_t b = sizeof(float4) * (size_t)maxN + big_queue_bytes(maxN) +
+               sizeof(double) * (4 * kBigWaves + 8) + sizeof(uint32_t) * (size_t)(maxW + 4);
+    return (b + 15) & ~(size_t)15;
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(kBigThreads)
+vf_step_f32_big_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order,
+                       const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
+                       uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
+                       unsigned long long* __restrict__ counters, float cull_c, int maxN, int maxR) {
+    extern __shared__ __align__(16) double smem_d[];
+    const VFRep* rp = reps + order[blockIdx.x];
+    const RepConst C = rep_const(rp);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    char* p = reinterpret_cast<char*>(smem_d);
+    double* red = reinterpret_cast<double*>(p); p += sizeof(double) * (4 * kBigWaves + 8);
+    float4* pose = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)maxN;
+    uint16_t* queue = reinterpret_cast<uint16_t*>(p); p += big_queue_bytes(maxN);
+    p += sizeof(int32_t) * 4;
+    uint32_t* F = reinterpret_cast<uint32_t*>(p);
+    // each wave queues the candidates among its own objects (64 of every
+    // 1024) in its own segment, so no counter is shared between waves
+    uint16_t* wq = queue + wave * big_segment(maxN);
+    for (int i = tid; i < C.N; i += kBigThreads) {
+        const int g = C.a0 + i;
+        pose[i] = make_float4(st.x[g], st.y[g], (float)st.radius[g], st.theta[g]);
+    }
+    // this thread's rays [rs, rs + rn) and their table entries
+    const int rs = tid * RPL;
+    const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
+    RayTab32 T[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int r = rs + i;
+        if (r < C.R) T[i] = tab32[rp->table_offset + r];
+        else T[i].wcos = T[i].wsin = T[i].cos = T[i].sin = 0.f;
+    }
+    const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
+    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    // wave 0 moves the agents: agent k's speed and parameters are loaded one
+    // agent ahead, so no global load waits on the id-order chain
+    float vel_next = 0.f, a0_next = a0d, b0_next = b0d, v0_next = v0d;
+    if (wave == 0) {
+        vel_next = st.v[C.a0];
+        if (st.alp0) a0_next = st.alp0[C.a0];
+        if (st.bet0) b0_next = st.bet0[C.a0];
+        if (st.v0) v0_next = st.v0[C.a0];
+    }
+    __syncthreads();
+    for (int k = 0; k < C.N; ++k) {
+        for (int w = tid; w < C.W; w += kBigThreads) F[w] = 0u;
+        const float vel_k = vel_next, alp0_k = a0_next, bet0_k = b0_next, v0_k = v0_next;
+        if (wave == 0 && k + 1 < C.N) {
+            const int gn = C.a0 + k + 1;
+            vel_next = st.v[gn];
+            if (st.alp0) a0_next = st.alp0[gn];
+            if (st.bet0) b0_next = st.bet0[gn];
+            if (st.v0) v0_next = st.v0[gn];
+        }
+        const float4 fp = pose[k];
+        const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
+        float sth, cth;
+        sincos_f32(fth, &sth, &cth);
+        const float u1x = cth, u1y = -sth;
+        __syncthreads();
+        // ---- cheap pass: the pairs inside the cut-off (or a torus guard band)
+        int nq = 0;   // wave-uniform
+        for (int base = tid - lane; base < C.N; base += kBigThreads) {
+            const int j = base + lane;
+            bool c = false;
+            if (j < C.N && j != k) {
+                const float4 o = pose[j];
+                if (!(o.x == fx && o.y == fy)) {
+                    float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
+                    bool near_wrap = false;
+                    if (C.torus) {
+                        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+                        near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
+                        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+                        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+                    }
+                    const float lim = o.z * cull_c;
+                    const bool culled = !near_wrap && fmaf(dy, dy, dx * dx) > lim * lim;
+                    if (culled && count) ++n_pairs;   // decided here: out of sight
+                    if (culled && verify) {
+                        // a culled pair must have an empty raster
+                        int es = 0, ee = 0, ed = 0;
+                        exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                        if (ed && es != ee) ++n_disagree;
+                    }
+                    c = !culled;
+                }
+            }
+            const uint64_t m = __ballot(c);
+            if (c) wq[nq + (int)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)j;
+            nq += (int)__popcll(m);
+        }
+        wave_lds_sync();
+        // ---- this wave's queued pairs: float32 decision, float64 inside the guard bands
+        for (int q = lane; q < nq; q += 64) {
+            const float4 o = pose[wq[q]];
+            PairF32 pr = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
+            if (count) ++n_pairs;
+            if (pr.tie && tie_in_place(pr, fx, fy, fth, fr, o.x, o.y, o.z, rp, C))   // rare
+                if (count) ++n_recheck;
+            if (!pr.ok || verify) {
+                int es = 0, ee = 0, ed = 0;
+                exact_pair(fx, fy, fth, fr, o.x, o.y, o.z, rp, &es, &ee, &ed);
+                if (verify && pr.ok && (es != pr.start || ee != pr.end || ed != (int)pr.drawn))
+                    ++n_disagree;
+                if (!pr.ok) {
+                    if (count) ++n_recheck;
+                    pr.start = es; pr.end = ee; pr.drawn = ed != 0;
+                }
+            }
+            if (pr.drawn) raster(F, pr.start, pr.end, C.R);
+        }
+        __syncthreads();
+        // ---- sums over this thread's rays
+        const bool case1 = (F[0] & 1u) == 1u && field_bit(F, C.R - 1) == 0u;
+        float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
+        if (rn > 0) {
+            const int w0 = rs >> 5;
+            const uint32_t lo = F[w0];
+            const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? F[w0 + 1] : 0u;
+            const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
+            const uint32_t m = low_mask(rn);
+            const uint32_t cur = (uint32_t)v64 & m;
+            uint32_t spike;
+            if (case1) {
+                const uint32_t prev = field_bit(F, rs == 0 ? C.R - 1 : rs - 1);
+                spike = (cur ^ ((cur << 1) | prev)) & m;
+            } else {
+                const int nx = rs + rn;
+                const uint32_t next = field_bit(F, nx >= C.R ? 0 : nx);
+                spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+            }
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) {
+                const bool vb = (cur >> i) & 1u;
+                const bool sb = (spike >> i) & 1u;
+                bc += vb ? T[i].wcos : 0.f;
+                bs += vb ? T[i].wsin : 0.f;
+                ec += sb ? T[i].cos : 0.f;
+                es += sb ? T[i].sin : 0.f;
+            }
+        }
+        const int gk = C.a0 + k;
+        const float alp0 = alp0_k, bet0 = bet0_k, v0 = v0_k;
+        // blob terms: trapz(cos(PHI) * (-V)) = -sum(w cos V); components by
+        // wave in LDS, summed in wave order by every thread
+        {
+            const double Bc = wave_total(bc), Bs = wave_total(bs), Ec = wave_total(ec), Es = wave_total(es);
+            if (lane == 0) {
+                red[4 * wave] = Bc; red[4 * wave + 1] = Bs; red[4 * wave + 2] = Ec; red[4 * wave + 3] = Es;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
+            for (int w = 0; w < kBigWaves; ++w) {
+                Bc += red[4 * w]; Bs += red[4 * w + 1]; Ec += red[4 * w + 2]; Es += red[4 * w + 3];
+            }
+            const double Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+            const double Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+            float vel, o, dv, dpsi;
+            steer(C, fth, vel_k, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
+            float so, co;
+            sincos_f32(o, &so, &co);
+            float nx, ny;
+            euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
+            if (lane == 0) {
+                pose[k] = make_float4(nx, ny, fr, o);
+                st.v[gk] = vel;
+                if (want_diag) {
+                    float* dd = diag + 6 * (int64_t)gk;
+                    dd[0] = dv; dd[1] = dpsi;
+                    dd[2] = C.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
+                    dd[3] = C.phi_ok ? (float)((double)(alp0 * C.alp1) * Ec) : 0.f;
+                    dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
+                    dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
+                }
+            }
+        }
+        if (bits_out) {
+            uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
+            for (int w = tid; w < C.W; w += kBigThreads) dst[w] = F[w];
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < C.N; i += kBigThreads) {
+        const int g = C.a0 + i;
+        const float4 q = pose[i];
+        st.x[g] = q.x;
+        st.y[g] = q.y;
+        st.theta[g] = q.w;
+    }
+    if (count) {
+        if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+    }
+}
+
+// The 16-wave kernel over the replicas order[0 .. n_rep), the largest with
+// maxN agents and maxR rays.
+template <int RPL>
+int launch_big(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+               uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, int maxN, int maxR) {
+    const size_t lds = lds_bytes_big(maxN, maxR);
+    if (lds > (size_t)b->dev_max_lds)
+        return set_error(ABM_ERR_UNSUPPORTED,
+                         "abm_vf_step_f32: %d agents in one replica need %zu B of LDS in the reference "
+                         "(sequential) order (device max %d); use ABM_ORDER_SYNCHRONOUS", maxN,
+                         lds, b->dev_max_lds);
+    auto kern = vf_step_f32_big_kernel<RPL>;
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    // the cut-off of the largest R: no smaller for a replica of fewer rays
+    // (cot(2 pi / R) grows with R), so it drops no drawable pair there either
+    const float cull_c = (float)((1.0 / std::tan(kTwoPi / maxR)) * (1.0 + 2e-4));
+    hipLaunchKernelGGL(kern, dim3(n_rep), dim3(kBigThreads), lds, s, b->reps_dev, order, b->tab32, *st,
+                       bits, diag, flags, b->counters, cull_c, maxN, maxR);
+    return check_hip(hipGetLastError(), "vf_step_f32_big_kernel launch");
+}
+
+int launch_big_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+                   uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, int maxN, int maxR) {
+    if (maxR <= kBigThreads) return launch_big<1>(b, st, bits, diag, flags, s, order, n_rep, maxN, maxR);
+    return launch_big<2>(b, st, bits, diag, flags, s, order, n_rep, maxN, maxR);
+}
+
 size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
     const int maxW = (maxR + 31) >> 5;
     size_t b = sizeof(RayTab32) * (size_t)rpl * 64 * nw + sizeof(float4) * (size_t)maxN * (sync_order ? 3 : 2) +
@@ -740,18 +1006,22 @@ size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
     return (b + 15) & ~(size_t)15;
 }
 
+// The per-replica kernel over the replicas order[0 .. n_rep), the largest
+// with maxN agents and maxR rays (split: the small size class of a batch
+// whose large class goes to the 16-wave kernel).
 template <int NW, int RPL>
 int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
-              uint32_t flags, hipStream_t s) {
+              uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, int maxN, int maxR,
+              bool split) {
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
-    const size_t lds = lds_bytes_f32(b->max_agents, b->max_rays, NW, RPL, sync_order);
+    const size_t lds = lds_bytes_f32(maxN, maxR, NW, RPL, sync_order);
     if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: %d agents x %d rays need %zu B of LDS (device max %d)",
-                         b->max_agents, b->max_rays, lds, b->dev_max_lds);
+                         maxN, maxR, lds, b->dev_max_lds);
     auto kern = vf_step_f32_kernel<NW, RPL>;
     const int slot = NW == 1 ? 0 : (NW == 2 ? 1 : 2);
-    int& per_cu = b->occ_f32[sync_order ? 1 : 0][slot];
+    int& per_cu = b->occ_f32[split ? 1 : 0][sync_order ? 1 : 0][slot];
     if (per_cu == 0) {   // first launch of this shape: set the LDS limit, read the occupancy
         if (lds > 65536) {
             if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
@@ -767,10 +1037,9 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
     }
     const int n_cu = b->n_cu;
     long grid = (long)per_cu * n_cu;
-    if (grid > b->n_replicas) grid = b->n_replicas;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NW), lds, s, b->reps_dev, b->order_dev,
-                       b->n_replicas, b->tab32, *st, bits, diag, flags, b->counters,
-                       b->max_agents);
+    if (grid > n_rep) grid = n_rep;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NW), lds, s, b->reps_dev, order,
+                       n_rep, b->tab32, *st, bits, diag, flags, b->counters, maxN);
     return check_hip(hipGetLastError(), "vf_step_f32_kernel launch");
 }
 
@@ -792,12 +1061,29 @@ int selected_group_size(const abm_vf_batch* b) {
 }
 
 // Waves per replica of the per-replica kernel when not set: as many (up to 4)
-// as leave room for all replicas at once (16 waves per CU).
-static int default_waves(const abm_vf_batch* b) {
+// as leave room for all its replicas at once (16 waves per CU).
+static int default_waves(const abm_vf_batch* b, int maxN, int n_rep) {
     for (int nw : {4, 2}) {
-        if (b->max_agents > 16 * nw && (long)b->n_replicas * nw <= 16L * b->n_cu) return nw;
+        if (maxN > 16 * nw && (long)n_rep * nw <= 16L * b->n_cu) return nw;
     }
     return 1;
+}
+
+// The per-replica kernel with nw waves (0 = automatic) over order[0 .. n_rep).
+static int launch_per_replica(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+                              uint32_t flags, hipStream_t s, int nw, const int32_t* order, int n_rep,
+                              int maxN, int maxR, bool split) {
+    if (nw == 0) nw = default_waves(b, maxN, n_rep);
+    const int need = (maxR + 64 * nw - 1) / (64 * nw);   // rays per lane
+    const int rpl = need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 16 ? 16 : 32));
+#define ABM_LAUNCH(NWV, RPLV) \
+    if (nw == NWV && rpl == RPLV) \
+        return launch_nw<NWV, RPLV>(b, st, bits, diag, flags, s, order, n_rep, maxN, maxR, split);
+    ABM_LAUNCH(1, 4) ABM_LAUNCH(1, 8) ABM_LAUNCH(1, 16) ABM_LAUNCH(1, 32)
+    ABM_LAUNCH(2, 4) ABM_LAUNCH(2, 8) ABM_LAUNCH(2, 16) ABM_LAUNCH(2, 32)
+    ABM_LAUNCH(4, 4) ABM_LAUNCH(4, 8) ABM_LAUNCH(4, 16) ABM_LAUNCH(4, 32)
+#undef ABM_LAUNCH
+    return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: no launch shape for %d waves", nw);
 }
 
 int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
@@ -806,12 +1092,14 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: n_rays up to 2048 (got %d); use abm_vf_step_f64", b->max_rays);
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
+    const bool kahan = (flags & ABM_KAHAN_SUMS) != 0;
+    if (kahan && b->max_agents > kWideThreshold)
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32: ABM_KAHAN_SUMS is implemented by the "
+                         "grouped kernel only (replicas of at most %d agents)", kGroupMaxAgents);
     if (b->max_agents > kWideThreshold) {
-        if (!sync_order)
-            return set_error(ABM_ERR_UNSUPPORTED,
-                             "abm_vf_step_f32: %d agents in one replica do not fit one CU's LDS; the "
-                             "reference (sequential) order is not available at this size, use "
-                             "ABM_ORDER_SYNCHRONOUS", b->max_agents);
+        if (!sync_order)   // reference order: one workgroup per replica, every pose in LDS
+            return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
+                                  b->max_rays);
         if (!b->wide_out)
             return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: batch has no wide-step buffer");
         const int need = (b->max_rays + 63) / 64;
@@ -820,19 +1108,32 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         if (need <= 16) return launch_wide<16>(b, st, bits, diag, flags, s);
         return launch_wide<32>(b, st, bits, diag, flags, s);
     }
-    const int G = selected_group_size(b);
+    // Kahan sums run on the grouped kernel even where the automatic choice
+    // (or a forced launch shape) would take the per-replica kernel
+    const int G0 = selected_group_size(b);
+    const int G = kahan && b->group_ok && G0 == 0 ? 1 : G0;
     if (G > 0) return launch_vf_group_f32(b, G, st, bits, diag, flags, s);
-    int nw = b->waves_per_group;
-    if (nw == 0) nw = default_waves(b);
-    const int need = (b->max_rays + 64 * nw - 1) / (64 * nw);   // rays per lane
-    const int rpl = need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 16 ? 16 : 32));
-#define ABM_LAUNCH(NWV, RPLV) \
-    if (nw == NWV && rpl == RPLV) return launch_nw<NWV, RPLV>(b, st, bits, diag, flags, s);
-    ABM_LAUNCH(1, 4) ABM_LAUNCH(1, 8) ABM_LAUNCH(1, 16) ABM_LAUNCH(1, 32)
-    ABM_LAUNCH(2, 4) ABM_LAUNCH(2, 8) ABM_LAUNCH(2, 16) ABM_LAUNCH(2, 32)
-    ABM_LAUNCH(4, 4) ABM_LAUNCH(4, 8) ABM_LAUNCH(4, 16) ABM_LAUNCH(4, 32)
-#undef ABM_LAUNCH
-    return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: no launch shape for %d waves", nw);
+    if (kahan)
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32: ABM_KAHAN_SUMS is implemented by the "
+                         "grouped kernel only, which this batch does not take (one R up to %d rays, "
+                         "replicas of at most %d agents)", kGroupMaxRays, kGroupMaxAgents);
+    const int nw = b->waves_per_group;
+    if (nw == 16)   // the 16-wave kernel for every replica
+        return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
+                              b->max_rays);
+    const int nb = b->n_big_class;
+    if (nw == 0 && nb > 0 && nb < b->n_replicas) {
+        // size classes: the small replicas on the per-replica kernel, then
+        // the large ones on the 16-wave kernel (their id-order chains are
+        // the step's critical path)
+        if (int rc = launch_per_replica(b, st, bits, diag, flags, s, 0, b->order_dev + nb,
+                                        b->n_replicas - nb, b->small_max_agents, b->small_max_rays, true))
+            return rc;
+        return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, nb, b->big_max_agents,
+                              b->big_max_rays);
+    }
+    return launch_per_replica(b, st, bits, diag, flags, s, nw, b->order_dev, b->n_replicas, b->max_agents,
+                              b->max_rays, false);
 }
 
 }  // namespace abm

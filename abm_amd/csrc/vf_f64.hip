@@ -62,10 +62,15 @@ __global__ void __launch_bounds__(256)
 vf_step_f64_kernel(const VFRep* __restrict__ reps, const double* __restrict__ tcos,
                    const double* __restrict__ tsin, const double* __restrict__ tdphi,
                    abm_vf_state_f64 st, uint32_t* __restrict__ bits_out,
-                   double* __restrict__ diag, uint32_t flags) {
+                   double* __restrict__ diag, uint32_t flags, unsigned long long* __restrict__ counters) {
 #pragma clang fp contract(off)
     extern __shared__ double smem[];
     const VFRep P = reps[blockIdx.x];
+    // near-tie margin count: pairs whose bin or half-width decision lies
+    // within 1e-12 of its tie, where the device's and the host's libm (acos,
+    // atan) may round the deciding value to opposite sides
+    const bool count = (flags & ABM_COUNT_RECHECKS) != 0 && counters != nullptr;
+    unsigned long long n_near = 0;
     const int N = P.n_agents, R = P.n_rays;
     const int tid = threadIdx.x, nthr = blockDim.x, nw = nthr >> 6;
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
@@ -104,6 +109,17 @@ vf_step_f64_kernel(const VFRep* __restrict__ reps, const double* __restrict__ tc
             if (j == k) continue;
             PairVF p = pair_vf_exact(f, ox[j], oy[j], srad[j], torus, P.width, P.height,
                                      R, P.step, P.fov_px0, P.fov_px1);
+            if (count && !p.skipped) {
+                const double ca = p.closed_angle;
+                const double dt = fabs(linspace_at(p.t, R, P.step) - ca);
+                double dn = 1e300;
+                if (p.t > 0) dn = fmin(dn, fabs(linspace_at(p.t - 1, R, P.step) - ca));
+                if (p.t < R - 1) dn = fmin(dn, fabs(linspace_at(p.t + 1, R, P.step) - ca));
+                const double hf = p.half_f, fr = hf - floor(hf);
+                const bool near_bin = dn - dt < 1e-12;
+                const bool near_half = hf >= 0.5 && (fr < 1e-12 * hf || 1.0 - fr < 1e-12 * hf);
+                if (near_bin || near_half) ++n_near;
+            }
             if (p.skipped || !p.drawn) continue;
             int a0p, b0p, a1p, b1p;
             vf_pieces(p.start, p.end, R, &a0p, &b0p, &a1p, &b1p);
@@ -164,6 +180,7 @@ vf_step_f64_kernel(const VFRep* __restrict__ reps, const double* __restrict__ tc
         st.theta[a0 + i] = sth[i];
         st.v[a0 + i] = sv[i];
     }
+    if (count && n_near) atomicAdd(&counters[3], n_near);
 }
 
 }  // namespace
@@ -191,7 +208,8 @@ int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bi
             return rc;
     }
     hipLaunchKernelGGL(vf_step_f64_kernel, dim3(b->n_replicas), dim3(nthr), lds, s,
-                       b->reps_dev, b->tab_cos, b->tab_sin, b->tab_dphi, *st, bits, diag, flags);
+                       b->reps_dev, b->tab_cos, b->tab_sin, b->tab_dphi, *st, bits, diag, flags,
+                       (unsigned long long*)b->counters);
     return check_hip(hipGetLastError(), "vf_step_f64_kernel launch");
 }
 

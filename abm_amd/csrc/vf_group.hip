@@ -88,7 +88,7 @@ __device__ inline void raster_pred(uint32_t* F, int start, int end, int R, bool 
     }
 }
 
-template <int G, int RPL>
+template <int G, int RPL, bool KH>
 __global__ void __launch_bounds__(64 * (G == 1 ? 8 : 16 / G))
 __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 : 1)))
 vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ groups, int n_groups,
@@ -242,8 +242,22 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
             }
             float bc, bs, ec, es;
+            float Sv, Sp, Bc, Bs, Ec, Es;   // group totals in diag mode
+            if (KH) {
+                // Kahan-compensated ray sums over the lane's nibble entries,
+                // then the group's sums of the compensated values in float64
+                float ks[4], kc[4];
+                span_sums_kahan<RPL, L>(tb, ts, v, sp, ks, kc);
+                const double dBc = group_sum_f64<G>((double)ks[0] - (double)kc[0]);
+                const double dBs = group_sum_f64<G>((double)ks[1] - (double)kc[1]);
+                const double dEc = group_sum_f64<G>((double)ks[2] - (double)kc[2]);
+                const double dEs = group_sum_f64<G>((double)ks[3] - (double)kc[3]);
+                Sv = (float)((double)fd.y * -dBc + (double)(fd.y * C.alp1) * dEc);
+                Sp = (float)((double)fd.z * -dBs + (double)(fd.z * C.bet1) * dEs);
+                Bc = (float)dBc; Bs = (float)dBs; Ec = (float)dEc; Es = (float)dEs;
+            } else {
             span_sums<RPL, L>(tb, ts, v, sp, &bc, &bs, &ec, &es);
-            float Sv, Sp, Bc = bc, Bs = bs, Ec = ec, Es = es;   // group totals in diag mode
+            Bc = bc; Bs = bs; Ec = ec; Es = es;
             if (!want_diag) {
                 Sv = fd.y * (-bc) + (fd.y * C.alp1) * ec;
                 Sp = fd.z * (-bs) + (fd.z * C.bet1) * es;
@@ -253,6 +267,7 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
                 group_sum2<G>(Ec, Es);
                 Sv = fd.y * -Bc + (fd.y * C.alp1) * Ec;
                 Sp = fd.z * -Bs + (fd.z * C.bet1) * Es;
+            }
             }
 
             // ------------------------------------------------ motion
@@ -298,7 +313,7 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
     }
 }
 
-template <int G, int RPL>
+template <int G, int RPL, bool KH>
 int launch_group_shape(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                        uint32_t flags, hipStream_t s) {
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
@@ -315,7 +330,7 @@ int launch_group_shape(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
     if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32 (grouped): %zu B of LDS needed (device max %d)",
                          lds, b->dev_max_lds);
-    auto kern = vf_group_kernel<G, RPL>;
+    auto kern = vf_group_kernel<G, RPL, KH>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -347,8 +362,11 @@ int group_rpl(int G, int R) {
 int launch_vf_group_f32(abm_vf_batch* b, int G, const abm_vf_state_f32* st, uint32_t* bits,
                         float* diag, uint32_t flags, hipStream_t s) {
     const int rpl = group_rpl(G, b->max_rays);
+    const bool kahan = (flags & ABM_KAHAN_SUMS) != 0;
 #define ABM_GLAUNCH(GV, RV) \
-    if (G == GV && rpl == RV) return launch_group_shape<GV, RV>(b, st, bits, diag, flags, s);
+    if (G == GV && rpl == RV) \
+        return kahan ? launch_group_shape<GV, RV, true>(b, st, bits, diag, flags, s) \
+                     : launch_group_shape<GV, RV, false>(b, st, bits, diag, flags, s);
     ABM_GLAUNCH(1, 4) ABM_GLAUNCH(1, 8) ABM_GLAUNCH(1, 16)
     ABM_GLAUNCH(2, 4) ABM_GLAUNCH(2, 8) ABM_GLAUNCH(2, 16) ABM_GLAUNCH(2, 32)
     ABM_GLAUNCH(4, 4) ABM_GLAUNCH(4, 8) ABM_GLAUNCH(4, 16) ABM_GLAUNCH(4, 32) ABM_GLAUNCH(4, 64)

@@ -209,5 +209,52 @@ __device__ inline void span_sums(const float2* tb, const float2* ts, uint64_t v,
     *bc = a; *bs = b; *ec = c; *es = d;
 }
 
+// Kahan-compensated version of span_sums: each of the four sums carries its
+// rounding error (Kahan's running compensation) over the lane's nibble
+// entries; the results are value + compensation pairs.
+template <int RPL, int L>
+__device__ inline void span_sums_kahan(const float2* tb, const float2* ts, uint64_t v, uint64_t sp,
+                                       float s[4], float c[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { s[i] = 0.f; c[i] = 0.f; }
+#pragma unroll
+    for (int q = 0; q < RPL / 4; ++q) {
+        const uint32_t vw = q < 8 ? (uint32_t)v : (uint32_t)(v >> 32);
+        const uint32_t sw = q < 8 ? (uint32_t)sp : (uint32_t)(sp >> 32);
+        const uint32_t vb = nibble(vw, 4 * (q & 7));
+        const uint32_t sb = nibble(sw, 4 * (q & 7));
+        const float2 tbv = (tb + q * 16 * L)[vb * L];
+        const float2 tsv = (ts + q * 16 * L)[sb * L];
+        const float x[4] = {tbv.x, tbv.y, tsv.x, tsv.y};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {   // Kahan step: y = x - c; t = s + y; c = (t - s) - y; s = t
+            const float y = x[i] - c[i];
+            const float t = s[i] + y;
+            c[i] = (t - s[i]) - y;
+            s[i] = t;
+        }
+    }
+}
+
+// Sum over each group of L = 64 / G lanes in float64 (the lane values are
+// exact in double): the DPP steps of group_sum2 on both halves of each
+// double; valid in the last lane of every group (every lane for G = 4).
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ inline double dpp_f64_any(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, BOUND);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, BOUND);
+    return __hiloint2double(hi, lo);
+}
+template <int G>
+__device__ inline double group_sum_f64(double x) {
+    x += dpp_f64_any<0xB1, 0xf, true>(x);    // quad_perm [1,0,3,2]
+    x += dpp_f64_any<0x4E, 0xf, true>(x);    // quad_perm [2,3,0,1]
+    x += dpp_f64_any<0x141, 0xf, true>(x);   // row_half_mirror
+    x += dpp_f64_any<0x140, 0xf, true>(x);   // row_mirror: every lane holds its row's sum
+    if (G <= 2) x += dpp_f64_any<0x142, 0xa, false>(x);   // row_bcast15 -> rows 1, 3
+    if (G == 1) x += dpp_f64_any<0x143, 0xc, false>(x);   // row_bcast31 -> rows 2, 3
+    return x;
+}
+
 }  // namespace f32
 }  // namespace abm

@@ -173,11 +173,14 @@ class VFSwarm:
                  alp0=p(self.alp0), bet0=p(self.bet0), v0=p(self.v0))
 
     def step(self, order: str = "sequential", fields: bool = False, diag: bool = False,
-             count: bool = False, verify: bool = False, stream=None):
+             count: bool = False, verify: bool = False, kahan: bool = False, stream=None):
         """Advance every replica by one step (one kernel launch).
 
-        ``count`` / ``verify`` (fused kernel only) keep the cumulative pair,
-        float64 re-check and float32-vs-float64 disagreement counters."""
+        ``count`` / ``verify`` (fused kernel) keep the cumulative pair,
+        float64 re-check and float32-vs-float64 disagreement counters;
+        ``count`` on the float64 kernel keeps its near-tie count.  ``kahan``
+        (fused kernel, batches the grouped kernel takes): Kahan-compensated
+        ray sums and group sums in float64."""
         if order not in ("sequential", "synchronous"):
             raise ValueError(order)
         flags = L.ORDER_SEQUENTIAL if order == "sequential" else L.ORDER_SYNCHRONOUS
@@ -187,6 +190,8 @@ class VFSwarm:
             flags |= L.COUNT_RECHECKS
         if verify:
             flags |= L.VERIFY_FAST_PATH
+        if kahan:
+            flags |= L.KAHAN_SUMS
         st = self._state()
         fb = L.ptr(self.fields) if fields else None
         dg = L.ptr(self.diag) if diag else None
@@ -202,6 +207,13 @@ class VFSwarm:
                                                          L.stream_ptr(stream)))
         return tot.value, rec.value, dis.value
 
+    def near_ties(self, stream=None):
+        """Pairs the float64 step decided within 1e-12 of a bin or half-width
+        tie, cumulative over steps run with ``count=True``."""
+        n = C.c_int64()
+        L.check(self.batch.lib.abm_vf_f64_near_ties(self.batch.handle, C.byref(n), L.stream_ptr(stream)))
+        return n.value
+
     def autotune(self, trial_steps=5, stream=None):
         if self.precision != "f32":
             raise ValueError("autotune applies to the fused float32 kernel")
@@ -212,7 +224,10 @@ class VFSwarm:
         return w.value, ms.value
 
     def set_launch(self, waves_per_group: int):
-        """Per-replica fused kernel with this many waves per replica (0 = automatic)."""
+        """Per-replica fused kernel with this many waves per replica (1, 2, 4),
+        the 16-wave kernel (one workgroup per replica) for every replica (16),
+        or the automatic choice (0: replicas above 128 agents on the 16-wave
+        kernel, the rest on the per-replica kernel)."""
         L.check(self.batch.lib.abm_vf_set_launch(self.batch.handle, waves_per_group))
 
     def set_grouping(self, replicas_per_wave: int):

@@ -58,10 +58,12 @@ def make_workload(n_rep_side=B_SIDE):
 
 def time_steps(sw, steps, warmup, use_graph=True, per_graph=10):
     """Average time (ms) of one step over `steps` steps, after `warmup` untimed
-    steps.  With graphs, `per_graph` consecutive steps are captured in one HIP
-    graph (the step kernels back to back, no host launch between them); each
-    replay is bracketed by events on the launch stream and its time divided by
-    `per_graph`.  Returns (mean, median, wall-clock per step)."""
+    steps.  With graphs, up to `per_graph` consecutive steps are captured in
+    one HIP graph (the step kernels back to back, no host launch between
+    them; fewer when that leaves at least five replays to take the spread
+    of); each replay is bracketed by events on the launch stream and its time
+    divided by its steps.  Returns (mean, median, wall-clock per step); the
+    quartiles of the per-replay times are left in ``time_steps.last_iqr``."""
     import torch
     for _ in range(warmup):
         sw.step()
@@ -72,7 +74,8 @@ def time_steps(sw, steps, warmup, use_graph=True, per_graph=10):
     if use_graph:
         # steps per graph: the largest divisor of `steps` up to per_graph, so
         # exactly `steps` steps are timed
-        per = max(d for d in range(1, max(1, min(per_graph, steps)) + 1) if steps % d == 0)
+        cap = max(1, min(per_graph, steps // 5 if steps >= 5 else steps))
+        per = max(d for d in range(1, cap + 1) if steps % d == 0)
         reps = max(1, steps // per)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -97,6 +100,8 @@ def time_steps(sw, steps, warmup, use_graph=True, per_graph=10):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     per_step = [s.elapsed_time(e) / per for s, e in zip(starts, ends)]
+    time_steps.last_iqr = [float(v) for v in np.percentile(per_step, [25, 75])]
+    time_steps.last_replays = (reps, per)
     return float(np.mean(per_step)), float(np.median(per_step)), wall * 1e3 / (reps * per)
 
 
@@ -138,10 +143,31 @@ def _cpu_worker(args):
     return steps * N / dt
 
 
+def host_cpu():
+    """os.cpu_count(), the cores this process may run on, and the CPU model
+    (from /proc/cpuinfo, the line lscpu reports as the model name)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_count": os.cpu_count(), "affinity_cores": aff, "model": model}
+
+
+# One GPU's share of the host: the GPU box gives each GPU 16 cores, whatever
+# os.cpu_count() reports there, so the baseline uses at most 16 processes.
+CPU_SHARE = 16
+
+
 def cpu_baseline(seconds=15.0):
     """The oracle on the host cores, one process per core, one replica each."""
     cores = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
-    cores = cores[:16]
+    cores = cores[:CPU_SHARE]
     # size the sample: calibrate one step on one core
     from oracle import vf_oracle as vo  # noqa: F401
     steps = max(2, int(seconds / 0.12))
@@ -150,11 +176,11 @@ def cpu_baseline(seconds=15.0):
         rates = pool.map(_cpu_worker, [(c, k * 61 % (B_SIDE * B_SIDE), steps)
                                        for k, c in enumerate(cores)])
     return {"value": float(sum(rates)), "unit": "agent-steps/s", "cores": len(cores),
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": (f"oracle (numpy restatement of the reference step), {len(cores)} processes "
-                       f"pinned one per core, one replica each (N={N}, R={R}, torus) for "
-                       f"{steps} steps after 1 warm-up; per-core rate {np.mean(rates):.1f} "
-                       f"agent-steps/s")}
+                       f"pinned one per core (at most {CPU_SHARE}: one GPU's share of the host), "
+                       f"one replica each (N={N}, R={R}, torus) for {steps} steps after 1 "
+                       f"warm-up; per-core rate {np.mean(rates):.1f} agent-steps/s")}
 
 
 def _line(args, workload, value, ms, dtype, config, **extra):
@@ -199,30 +225,62 @@ def other_workload(args):
         th = f32(rng.uniform(0, 2 * np.pi, Nl))
         cfg = VFConfig(n_agents=Nl, n_rays=Rl, width=Wl, height=Wl, boundary="infinite", ALP0=1.0,
                        BET0=0.5)
+        cpu = None if args.no_cpu else large_cpu_sample(x, y, th, Nl, Rl, Wl, min(args.cpu_seconds, 10.0))
         sw = VFSwarm([cfg], x, y, th, 0.5, radius=2, precision="f32")
 
         class Sync:
             step = staticmethod(lambda: sw.step(order="synchronous"))
         ms = time_steps(Sync, args.steps, args.warmup, not args.no_graph)[0]
-        ops = ops_per_agent_step(Nl, Rl) * Nl
-        _line(args, f"one flock N={Nl}, R={Rl}, radius 2, 4000x4000 torus, synchronous order (wide kernel)",
-              Nl / (ms * 1e-3), ms, "f32", {"replicas": 1, "agents_per_replica": Nl, "rays": Rl,
-                                             "order": "synchronous"},
-              roofline={"bound": "valu", "achieved": ops / (ms * 1e-3) / 1e12, "peak": VALU_PEAK_TOPS,
-                        "unit": "Tops/s", "frac": ops / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-                        "traffic": None,
-                        "note": "nominal all-pairs ops; pairs beyond the exact visibility cut-off are dropped"},
-              pair_ray_evals_per_s=Nl * (Nl - 1) * Rl / (ms * 1e-3))
+        iqr = getattr(time_steps, "last_iqr", None)
+        # the reference's order: one workgroup, agents in id order
+        sq = VFSwarm([cfg], x, y, th, 0.5, radius=2, precision="f32")
+        k_seq = max(5, min(args.steps // 5, 40))
+        ms_seq = time_steps(sq, k_seq, min(args.warmup, 5), not args.no_graph)[0]
+        iqr_seq = getattr(time_steps, "last_iqr", None)
+        # pairs a step must decide: those inside the exact visibility cut-off
+        # r cot(2 pi / R) (beyond it floor(proj_size / 2) = 0 and nothing is
+        # drawn, vf_supcalc.py:116-117), torus images, from the start state
+        cut = 2.0 / np.tan(2 * np.pi / Rl)
+        eff = 0
+        for i0 in range(0, Nl, 512):
+            dx = np.abs(x[i0:i0 + 512, None] - x[None, :])
+            dy = np.abs(y[i0:i0 + 512, None] - y[None, :])
+            dx = np.minimum(dx, Wl - dx)
+            dy = np.minimum(dy, Wl - dy)
+            eff += int(((dx * dx + dy * dy) <= cut * cut).sum())
+        eff -= Nl                                   # each agent with itself
+        ops = 40 * eff + (17 * Rl + 20) * Nl
+        ach = ops / (ms_seq * 1e-3) / 1e12
+        ach_sync = ops / (ms * 1e-3) / 1e12
+        note = ("lane-ops of the pairs inside the visibility cut-off (40 each) and of every agent's "
+                "rays (17 each) and update (20)")
+        extra = {"cpu_baseline": cpu} if cpu is not None else {}
+        _line(args, f"one flock N={Nl}, R={Rl}, radius 2, 4000x4000 torus, reference order (one "
+                    f"workgroup, agents in id order)",
+              Nl / (ms_seq * 1e-3), ms_seq, "f32", {"replicas": 1, "agents_per_replica": Nl, "rays": Rl,
+                                                     "order": "sequential"},
+              ms_per_step_iqr=iqr_seq,
+              roofline={"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                        "frac": ach / VALU_PEAK_TOPS, "traffic": None,
+                        "note": note + "; one workgroup runs the id-order chain: latency-bound"},
+              synchronous_order={"agent_steps_per_s": Nl / (ms * 1e-3), "ms_per_step": ms,
+                                 "ms_per_step_iqr": iqr, "kernel": "wide (grid cells, one wave per agent)",
+                                 "roofline_frac": ach_sync / VALU_PEAK_TOPS,
+                                 "note": "not the reference's order: every field from the step's "
+                                         "start poses"},
+              effective_pairs_per_step=eff, nominal_pairs_per_step=Nl * (Nl - 1),
+              pair_ray_evals_per_s=eff * Rl / (ms_seq * 1e-3),
+              nominal_pair_ray_evals_per_s=Nl * (Nl - 1) * Rl / (ms_seq * 1e-3), **extra)
     elif args.workload == "cs":
         return cs_workload(args)
     elif args.workload == "c1":
         return c1_workload(args)
+    elif args.workload == "c5overlay":
+        return c5_overlay_workload(args)
     else:   # mixed ragged batch: even replicas forage, odd replicas flock
-        rng = np.random.RandomState(20000)
+        cpu = None if args.no_cpu else mixed_cpu_baseline(args.cpu_seconds)
         vf_cfgs, fg_cfgs, vx, fx_ = [], [], [], []
-        for b in range(2000):
-            n = int(round(np.exp(rng.uniform(np.log(8), np.log(512)))))
-            r = int(rng.choice([256, 512, 1024]))
+        for b, (n, r) in enumerate(mixed_shapes()):
             side = 100.0 * np.sqrt(n)
             srng = np.random.RandomState(20000 + b)
             pos = (f32(srng.uniform(PAD, PAD + side, n)), f32(srng.uniform(PAD, PAD + side, n)),
@@ -261,13 +319,109 @@ def other_workload(args):
         n_vf = int(vsw.batch.info.n_agents_total)
         n_fg = int(fsw.info.n_agents_total)
         ms = ms_vf + ms_fg
+        ops_vf = sum(c.n_agents * ops_per_agent_step(c.n_agents, c.n_rays) for c in vf_cfgs)
+        ach = ops_vf / (ms_vf * 1e-3) / 1e12
+        extra = {"roofline": {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                              "frac": ach / VALU_PEAK_TOPS, "traffic": None,
+                              "kernel": "flocking half (per-replica fused kernel), lane-ops of every "
+                                        "flocking agent-step over its graph-replay time"}}
+        if cpu is not None:
+            extra["cpu_baseline"] = cpu
         _line(args, "2000 ragged replicas (N log-uniform 8..512, R in {256,512,1024}, side 100*sqrt(N)): "
                     "odd replicas flock (fused float32), even replicas forage (float64) among 10 "
                     "resource patches each (interaction, regeneration and update on the device)",
               (n_vf + n_fg) / (ms * 1e-3), ms, "f32+f64",
               {"replicas": 2000, "agents_flocking": n_vf, "agents_foraging": n_fg},
-              flocking={"agent_steps_per_s": n_vf / (ms_vf * 1e-3), "ms_per_step": ms_vf},
-              foraging={"agent_steps_per_s": n_fg / (ms_fg * 1e-3), "ms_per_step_wall": ms_fg})
+              flocking={"agent_steps_per_s": n_vf / (ms_vf * 1e-3), "ms_per_step": ms_vf,
+                        "ms_per_step_iqr": getattr(time_steps, "last_iqr", None),
+                        "launch": vsw.chosen_launch if hasattr(vsw, "chosen_launch") else None},
+              foraging={"agent_steps_per_s": n_fg / (ms_fg * 1e-3), "ms_per_step_wall": ms_fg},
+              **extra)
+
+
+def large_cpu_sample(x, y, th, N, R, W, seconds):
+    """The oracle on one core for the 8192-agent flock: VFAgent.update of a
+    sample of agents (each sees all 8191 others), scaled to a step.  The
+    reference itself cannot hold this flock: its field is an (N-1) x R
+    float64 matrix per agent (vf_supcalc.py:38), 134 MB here."""
+    from oracle import vf_oracle as vo
+    p = vo.VFParams(ALP0=1.0, BET0=0.5, BOUNDARY="infinite")
+    v = np.full(N, 0.5)
+    t0, n = time.perf_counter(), 0
+    while n < 1 or (time.perf_counter() - t0 < seconds and n < 50):
+        vo.agent_view_step(x, y, th, v, 2, (n * 997) % N, R, (-np.pi, np.pi), W, W, PAD, p)
+        n += 1
+    per = (time.perf_counter() - t0) / n
+    return {"value": 1.0 / per, "unit": "agent-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle VFAgent.update of {n} agents of the 8192-agent flock ({per:.2f} s "
+                      f"each, one core); a whole step would take {per * N / 3600:.1f} h"}
+
+
+def mixed_shapes():
+    """(N, R) of the 2000 ragged replicas: N log-uniform in [8, 512], R in
+    {256, 512, 1024}; even replicas forage, odd replicas flock."""
+    rng = np.random.RandomState(20000)
+    out = []
+    for b in range(2000):
+        n = int(round(np.exp(rng.uniform(np.log(8), np.log(512)))))
+        r = int(rng.choice([256, 512, 1024]))
+        out.append((n, r))
+    return out
+
+
+def _mixed_cpu_worker(args):
+    core, b, seconds = args
+    try:
+        os.sched_setaffinity(0, {core})
+    except (AttributeError, OSError):
+        pass
+    n, r = mixed_shapes()[b]
+    side = 100.0 * np.sqrt(n)
+    srng = np.random.RandomState(20000 + b)
+    x, y, th = srng.uniform(PAD, PAD + side, n), srng.uniform(PAD, PAD + side, n), srng.uniform(0, 2 * np.pi, n)
+    if b % 2:
+        from oracle import vf_oracle as vo
+        p = vo.VFParams(ALP0=1.0, BET0=0.5, BOUNDARY="infinite")
+        ags = vo.make_agents(x, y, th, np.full(n, 0.5), 5, r, (-np.pi, np.pi), side, side, PAD, p)
+        step = lambda: vo.group_update(ags, p)
+    else:
+        from oracle import forage_oracle as fo
+        fp = fo.ForageParams()
+        ags = [fo.ForageAgent(i, 10, (x[i], y[i]), th[i], r, (-np.pi, np.pi), side, side, PAD, 2000.0,
+                              False, True, fp) for i in range(n)]
+        np.random.seed(30000 + b)
+        world = fo.ForageWorld(ags, side, side, PAD, 10, 30, 200, 1000, 0.5, 1.0, teleport_exploit=False)
+        step = world.step
+    t0, k = time.perf_counter(), 0
+    while k < 1 or (time.perf_counter() - t0 < seconds and k < 50):
+        step()
+        k += 1
+    return n, (time.perf_counter() - t0) / k
+
+
+def mixed_cpu_baseline(seconds=15.0):
+    """The oracle on the uneven batch: one replica per core (at most one
+    GPU's share of the host), sampled across the N / R mix (8 flocking and 8
+    foraging replicas spread over the sorted costs), each for at least one
+    step; the rate is sum(N) / sum(seconds per step) per core, times the
+    cores."""
+    cores = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    cores = cores[:CPU_SHARE]
+    shapes = mixed_shapes()
+    pick = []
+    for parity in (1, 0):
+        idx = sorted((b for b in range(len(shapes)) if b % 2 == parity),
+                     key=lambda b: shapes[b][0] * shapes[b][0] + shapes[b][0] * shapes[b][1])
+        half = len(cores) // 2
+        pick += [idx[int((i + 0.5) * len(idx) / half)] for i in range(half)]
+    with mp.get_context("spawn").Pool(len(cores)) as pool:
+        res = pool.map(_mixed_cpu_worker, [(c, b, seconds / 3) for c, b in zip(cores, pick)])
+    per_core = sum(n for n, _ in res) / sum(t for _, t in res)
+    return {"value": per_core * len(cores), "unit": "agent-steps/s", "cores": len(cores), "kind": "port",
+            "host": host_cpu(),
+            "sample": f"oracle (VSWRM group_update / ForageWorld.step) on {len(pick)} replicas of the "
+                      f"batch spread over its cost range, one per core, at least one step each; "
+                      f"ratio estimate sum(N) / sum(s per step) = {per_core:.1f} agent-steps/s per core"}
 
 
 def cs_workload(args):
@@ -409,6 +563,110 @@ def c1_workload(args):
                "(written back to numpy after the timed steps)")
 
 
+C5_WORLD = dict(N_resc=64, patch_radius=30, min_resc_perpatch=200, max_resc_perpatch=1000,
+                min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+
+
+def _c5_start(seed, N=8192, W=4000, r=2):
+    """Simulation.create_agents (sims.py:526-532) for the large population:
+    randint positions over [pad - r, W + pad - r), uniform orientations."""
+    g = np.random.RandomState(seed)
+    x, y, th = [], [], []
+    for _ in range(N):
+        x.append(float(g.randint(PAD - r, W + PAD - r)))
+        y.append(float(g.randint(PAD - r, W + PAD - r)))
+        th.append(g.uniform(0, 2 * np.pi))
+    return g, x, y, th
+
+
+def c5_cpu_sample(N, R, W, r, seconds):
+    """The oracle on one core for the large population: Agent.update of a
+    sample of agents inside the full 8192-agent world plus one resource
+    interaction pass, scaled to a whole step (one reference step would take
+    minutes)."""
+    from oracle import forage_oracle as fo
+    p = fo.ForageParams(**C1_ENV)
+    g, x, y, th = _c5_start(7, N, W, r)
+    ags = [fo.ForageAgent(i, r, (x[i], y[i]), th[i], R, (-np.pi, np.pi), W, W, PAD, 2000.0, False,
+                          True, p, pooling_time=0) for i in range(N)]
+    np.random.set_state(g.get_state())
+    world = fo.ForageWorld(ags, W, W, PAD, 64, 30, 200, 1000, 0.5, 1.0, regenerate=True,
+                           teleport_exploit=False)
+    # the interaction's membership pass: every patch against every agent
+    t0 = time.perf_counter()
+    groups = [[a for a in ags if fo.distance(q, a) < q.radius] for q in world.patches]
+    t_inter = time.perf_counter() - t0
+    del groups
+    t0, n = time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds and n < N:
+        fo.agent_update(ags[(n * 997) % N], ags)
+        n += 1
+    t_agent = (time.perf_counter() - t0) / n
+    step_s = N * t_agent + t_inter
+    return {"value": N / step_s, "unit": "agent-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle Agent.update of {n} agents sampled from the 8192-agent world "
+                      f"({t_agent * 1e3:.1f} ms each) plus one membership pass of the resource "
+                      f"loop ({t_inter:.1f} s), scaled to one step ({step_s:.0f} s) on one core"}
+
+
+def c5_overlay_workload(args):
+    """The foraging overlay of the large flock: the collective-foraging loop
+    body (resource interaction and regeneration, sims.py:789-858, then
+    Agent.update in id order, sims.py:861) for one population of N=8192
+    agents of radius 2, R=2048 over 2 pi, on the 4000x4000 arena with walls
+    (the foraging agent's only boundary, agent.py:279), 64 resource patches
+    of radius 30 with quality U[0.5, 1] and 200..1000 units, the C1 decision
+    values, vision range 2000, generators on the device.  The replica
+    exceeds one workgroup's LDS, so the update runs by sweeps."""
+    import torch
+    from abm_amd.forage import ForageConfig, ForageSwarm, ResourceWorld, WorldConfig
+    N, R, W, r = 8192, 2048, 4000, 2
+    cpu = None if args.no_cpu else c5_cpu_sample(N, R, W, r, min(args.cpu_seconds, 10.0))
+    cfg = ForageConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=PAD, vision_range=2000.0,
+                       **C1_ENV)
+    g, x, y, th = _c5_start(7, N, W, r)
+    sw = ForageSwarm([cfg], x, y, th, radius=r, rngs=[g])
+    rw = ResourceWorld(sw, [WorldConfig(**C5_WORLD)], device_rng=True)
+    rw.create_resources()
+    for _ in range(max(3, min(args.warmup, 20))):
+        rw.step()
+    torch.cuda.synchronize()
+    s0, w0 = sw.sweep_counts()
+    k = max(5, min(args.steps, 100))
+    times = []
+    for _ in range(k):
+        t0 = time.perf_counter()
+        rw.step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    s1, w1 = sw.sweep_counts()
+    rw.sync_rng()
+    ms = float(np.mean(times)) * 1e3
+    q25, q50, q75 = (float(v) * 1e3 for v in np.percentile(times, [25, 50, 75]))
+    st = sw.get_state("overriding_mode", "exploited_patch_id")
+    expl = int(((st["overriding_mode"] == 1) & (st["exploited_patch_id"] != -1)).sum())
+    # algorithmic work of one agent-step without visual exclusion: a record of
+    # each exploiting cue (40 lane-ops, as a pair of the flocking model), the
+    # raster and counts (3 per ray), the decision and move (60)
+    ops = N * (40 * expl + 3 * R + 60)
+    achieved = ops / (ms * 1e-3) / 1e12
+    extra = {"roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
+                          "unit": "Tops/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                          "note": "latency-bound: a few sweeps of small launches per step, "
+                                  "host-synchronised once per round; ops counted once per "
+                                  "agent-step with the exploiting cues at the end of the run"}}
+    if cpu is not None:
+        extra["cpu_baseline"] = cpu
+    _line(args, f"foraging overlay of the large flock: N={N} agents of radius 2, R={R}, 4000x4000 "
+                f"walls, 64 patches (radius 30, quality U[0.5, 1]), resource loop and Agent.update "
+                f"in id order (sweeps) on the device",
+          N / (ms * 1e-3), ms, "f64", {"replicas": 1, "agents_per_replica": N, "rays": R,
+                                        "patches": 64},
+          ms_per_step_median=q50, ms_per_step_iqr=[q25, q75],
+          sweeps_per_step=(w1 - w0) / max(1, s1 - s0), exploiting_cues_at_end=expl,
+          note="wall time per step (the sweeps read a few control words per round)", **extra)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,14 +677,15 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="sweep",
-                    choices=["sweep", "single", "mixed", "large", "cs", "c1"],
+                    choices=["sweep", "single", "mixed", "large", "cs", "c1", "c5overlay"],
                     help="sweep: 4096 replicas x N=50 x R=512 (default, the headline); single: "
                          "one replica N=100, R=1024, 1000x1000 torus; mixed: 2000 ragged replicas, "
                          "half foraging, half flocking; large: one flock N=8192, R=2048, "
                          "4000x4000 torus, synchronous order; cs: cooperative-signalling "
                          "perception of 1000 replicas x N=50 x R=1200; c1: the reference's CPU "
                          "foraging case (N=10, R=1200, 10 patches) on the device, with the "
-                         "oracle timed on one core")
+                         "oracle timed on one core; c5overlay: the foraging overlay of the "
+                         "large flock (N=8192, R=2048, 64 patches, 4000x4000)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -462,11 +721,11 @@ def main():
     # fixed error: one step of each kernel from the same state in synchronous
     # order, where every agent sees the same (pre-step) poses in both kernels,
     # so the fields must be identical and the pose error is the float32 error
-    def one_step(order):
+    def one_step(order, kahan=False):
         s64 = VFSwarm(cfgs, x, y, th, 0.5, radius=RADIUS, precision="f64")
         s32 = VFSwarm(cfgs, x, y, th, 0.5, radius=RADIUS, precision="f32")
-        s64.step(order=order, fields=True)
-        s32.step(order=order, fields=True, count=True)
+        s64.step(order=order, fields=True, count=True)
+        s32.step(order=order, fields=True, count=True, kahan=kahan)
         torch.cuda.synchronize()
         a64, a32 = s64.state_numpy(), s32.state_numpy()
         dx = np.abs((a32[0] - a64[0] + W / 2) % W - W / 2) / np.maximum(np.abs(a64[0]), 1.0)
@@ -479,18 +738,28 @@ def main():
         f32b = s32.fields.cpu().numpy().reshape(-1, words)
         same = (f64b == f32b).all(axis=1)
         pairs, rechecked, _ = s32.fallback_counts()
+        one_step.near_ties = s64.near_ties()
+        one_step.pairs = pairs
         return e, same, rechecked / max(pairs, 1)
 
     e_sync, same_sync, recheck_frac = one_step("synchronous")
+    near_ties, near_pairs = one_step.near_ties, one_step.pairs
     e_seq, same_seq, _ = one_step("sequential")
+    e_kahan, same_kahan, _ = one_step("synchronous", kahan=True)
     del one_step
 
     # timing
     sw32 = VFSwarm(cfgs, x, y, th, 0.0, radius=RADIUS, precision="f32")
     sw64 = VFSwarm(cfgs, x, y, th, 0.0, radius=RADIUS, precision="f64")
     ms32_mean, ms32_med, wall32 = time_steps(sw32, args.steps, args.warmup, not args.no_graph)
+    iqr32, replays32 = getattr(time_steps, "last_iqr", None), getattr(time_steps, "last_replays", (0, 0))
     ms64_mean, ms64_med, wall64 = time_steps(sw64, args.steps_f64, min(args.warmup, 3),
                                              not args.no_graph)
+    swk = VFSwarm(cfgs, x, y, th, 0.0, radius=RADIUS, precision="f32")
+
+    class KahanStep:
+        step = staticmethod(lambda: swk.step(kahan=True))
+    ms_kahan = time_steps(KahanStep, min(args.steps, 50), min(args.warmup, 10), not args.no_graph)[0]
     rate32 = agents / (ms32_mean * 1e-3)
     rate64 = agents / (ms64_mean * 1e-3)
     ops = ops_per_agent_step(N, R) * agents
@@ -527,9 +796,23 @@ def main():
             "sequential_order_rel_err_max_where_fields_identical":
                 float(e_seq[same_seq].max()) if same_seq.any() else None,
             "f64_rechecked_pair_fraction": recheck_frac,
+            "f64_near_tie_pairs": near_ties,
+            "f64_near_tie_note": (f"pairs of the float64 step (of {near_pairs}) whose bin or "
+                                  f"half-width decision lies within 1e-12 of its tie, where device "
+                                  f"and host libm may round to opposite sides"),
+        },
+        "kahan_sums": {
+            "one_step_rel_err_vs_f64": float(e_kahan.max()),
+            "fields_bit_exact_vs_f64": bool(same_kahan.all()),
+            "ms_per_step": ms_kahan,
+            "time_ratio_vs_default": ms_kahan / ms32_mean,
+            "note": "ABM_KAHAN_SUMS: Kahan-compensated ray sums per lane, group sums in float64 "
+                    "(the default: nibble-table sums and a float32 DPP tree)",
         },
         "pair_ray_evals_per_s": agents * (N - 1) * R / (ms32_mean * 1e-3),
         "ms_per_step_median": ms32_med,
+        "ms_per_step_iqr": iqr32,
+        "graph_replays": {"replays": replays32[0], "steps_per_replay": replays32[1]},
         "wall_ms_per_step": wall32,
         "roofline": {
             "bound": "valu",

@@ -60,6 +60,9 @@ extern "C" {
 #define ABM_COUNT_RECHECKS    4u  /* fused step: count pairs and float64 re-checks       */
 #define ABM_VERIFY_FAST_PATH  8u  /* fused step: also decide every pair in float64 and
                                      count float32 decisions that differ (testing)     */
+#define ABM_KAHAN_SUMS       16u  /* fused step, grouped kernel: Kahan-compensated ray
+                                     sums, group sums in float64 (ABM_ERR_UNSUPPORTED
+                                     for batches the grouped kernel does not take)      */
 
 /* ----------------------------------------------------- replica description */
 /*
@@ -150,14 +153,28 @@ int abm_vf_step_f32(abm_vf_batch* batch, const abm_vf_state_f32* state,
 int abm_vf_f32_fallback_count(abm_vf_batch* batch, int64_t* pairs_total,
                               int64_t* pairs_rechecked, int64_t* pairs_disagree, void* stream);
 
+/* Pairs the plain float64 step decided within 1e-12 of a tie -- the nearest
+ * linspace bin within 1e-12 rad of the second nearest, or proj_size / 2
+ * within 1e-12 relative of an integer -- where the device's libm (acos, atan)
+ * and the host's may round the deciding value to opposite sides; cumulative
+ * over the float64 steps run with ABM_COUNT_RECHECKS.  Synchronises the
+ * stream.  No counterpart in the reference (a diagnostic of the parity
+ * claim). */
+int abm_vf_f64_near_ties(abm_vf_batch* batch, int64_t* near_ties, void* stream);
+
 /* Launch shapes of the fused step (no counterpart in the reference, which
  * has no kernels).  Two kernels implement it: the per-replica kernel, with
  * 1, 2 or 4 waves cooperating on one replica, and the grouped kernel, with
  * 1, 2 or 4 replicas advanced in lockstep by one wave (for batches with one
  * n_rays <= 1024 and at most 256 agents per replica; replicas that differ in
- * more than N and the ALP0/BET0/V0 defaults never share a wave).
+ * more than N and the ALP0/BET0/V0 defaults never share a wave).  A third,
+ * the 16-wave kernel, gives one workgroup of 1024 threads to each replica
+ * (every pose in LDS); it runs single flocks above 2048 agents in the
+ * reference order and, by default, the large size class of a ragged batch
+ * (replicas above 128 agents) next to the per-replica kernel for the rest.
  * abm_vf_set_launch selects the per-replica kernel with waves_per_group
- * waves per replica (0 = automatic choice). */
+ * waves per replica (1, 2, 4), the 16-wave kernel for every replica (16),
+ * or the automatic choice (0: size classes when the batch has both). */
 int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
 /* replicas_per_wave: 1, 2 or 4 = grouped kernel with that many replicas per
  * wave (ABM_ERR_UNSUPPORTED when the batch does not qualify), -1 = per-replica
@@ -165,8 +182,9 @@ int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
 int abm_vf_set_grouping(abm_vf_batch* batch, int32_t replicas_per_wave);
 /* Time every candidate launch shape on the given state (the state is
  * advanced by the trial steps) and keep the fastest.  *chosen: waves per
- * replica (1, 2, 4) of the per-replica kernel, or -G for the grouped kernel
- * with G replicas per wave. */
+ * replica (1, 2, 4) of the per-replica kernel, 16 for the 16-wave kernel,
+ * 0 for the size classes, or -G for the grouped kernel with G replicas per
+ * wave. */
 int abm_vf_autotune(abm_vf_batch* batch, const abm_vf_state_f32* state, int32_t trial_steps,
                     int32_t* chosen_waves, float* chosen_ms, void* stream);
 

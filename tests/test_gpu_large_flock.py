@@ -1,7 +1,11 @@
-"""Single large flocks (more agents than one CU's LDS holds) take the wide
-synchronous kernel.  Checks: every float32 decision and every pair dropped
-by the visibility cut-off agree with float64 (verify counters), and sampled
-agents match the oracle on the pre-step poses (synchronous order)."""
+"""Single large flocks (more agents than the per-replica kernel holds).
+
+Reference (id) order runs one workgroup per replica with every pose in LDS
+(vf_sims.py:302 -> vf_agent.py:52-77): every float32 decision and every pair
+dropped by the visibility cut-off agree with float64 (verify counters), and
+sampled agents match the oracle on the poses they saw (agents < k moved).
+Synchronous order takes the wide kernel: the same checks on the pre-step
+poses."""
 import numpy as np
 import pytest
 
@@ -21,11 +25,30 @@ def flock(N, R, W, seed, r=2):
     return cfg, (x, y, th, v), VFSwarm([cfg], x, y, th, v, radius=r, precision="f32")
 
 
-def test_reference_order_is_refused_for_a_large_flock(gpu):
-    from abm_amd import AbmUnsupported
-    cfg, st, sw = flock(2100, 512, 1000, 1)
-    with pytest.raises(AbmUnsupported):
-        sw.step(order="sequential")
+def test_reference_order_large_flock_matches_oracle(gpu):
+    # N = 2500 > 2048, R = 2048, radius 2, 2400 px torus: one sequential step
+    # in verify mode, then every sampled agent's field and pose against the
+    # oracle on the poses that agent saw
+    from test_gpu_vf_f32 import per_agent_check
+    cfg, st, sw = flock(2500, 2048, 2400, 3)
+    sw.step(order="sequential", fields=True, diag=True, verify=True)
+    pairs, rechecked, disagree = sw.fallback_counts()
+    assert disagree == 0 and pairs == 2500 * 2499
+    agents = sorted({0, 1, 1249, 2498, 2499} | set(np.random.RandomState(1).choice(2500, 12, replace=False).tolist()))
+    per_agent_check(sw, 0, cfg, st, sw.state_numpy(), radius=2, agents=agents)
+
+
+def test_reference_order_c5_flock_sampled_agents(gpu):
+    # the large flock itself (N = 8192, R = 2048, 4000 px torus) in the
+    # reference order for two steps; on the second, sampled agents against
+    # the oracle on the poses they saw
+    from test_gpu_vf_f32 import per_agent_check
+    cfg, st, sw = flock(8192, 2048, 4000, 11)
+    sw.step(order="sequential")
+    before = sw.state_numpy()
+    sw.step(order="sequential", fields=True, diag=True)
+    agents = sorted({0, 4096, 8191} | set(np.random.RandomState(2).choice(8192, 5, replace=False).tolist()))
+    per_agent_check(sw, 0, cfg, before, sw.state_numpy(), radius=2, agents=agents)
 
 
 def test_large_flock_sampled_agents_match_oracle(gpu):
