@@ -1142,7 +1142,7 @@ struct SweepX {
 
 // Sweep control words.
 constexpr int kSwConverged = 0, kSwSkip = 1, kSwCand = 2, kSwIters = 3, kSwDraws = 4,
-              kSwUsed0 = 5, kSwPos0 = 6, kSwBad = 7, kSwWords = 8;
+              kSwUsed0 = 5, kSwPos0 = 6, kSwBad = 7, kSwTwists = 8, kSwWords = 9;
 
 constexpr int kSweepMaxWords = 256;   // R <= 8192: four field words per lane
 
@@ -1180,14 +1180,14 @@ sweep_prep_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
                   const int64_t* __restrict__ u_off, const int32_t* __restrict__ u_cnt,
                   const int32_t* __restrict__ u_used, const int32_t* __restrict__ status,
                   const abm_mt19937_state* __restrict__ mt, SweepX X0, double* __restrict__ uni,
-                  int32_t* __restrict__ cand, int32_t* __restrict__ ctrl) {
+                  uint32_t* __restrict__ keys, int32_t* __restrict__ cand, int32_t* __restrict__ ctrl) {
     __shared__ uint32_t key[kMtN];
     const FGRep& P = reps[rb];
     const int N = P.n_agents, a0 = P.agent_offset, tid = threadIdx.x, nthr = blockDim.x;
     const bool skip = status && status[rb] != kWorldReady;
     if (tid == 0) {
         ctrl[kSwConverged] = 0; ctrl[kSwSkip] = skip ? 1 : 0; ctrl[kSwIters] = 0;
-        ctrl[kSwDraws] = 0; ctrl[kSwBad] = 0;
+        ctrl[kSwDraws] = 0; ctrl[kSwBad] = 0; ctrl[kSwTwists] = 0;
         ctrl[kSwUsed0] = (words || mt) ? u_used[rb] : 0;
         ctrl[kSwPos0] = mt ? mt[rb].pos : 0;
     }
@@ -1206,10 +1206,14 @@ sweep_prep_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
         for (int i = tid; i < kMtN; i += nthr) key[i] = mt[rb].key[i];
         __syncthreads();
         uint32_t* wb = reinterpret_cast<uint32_t*>(uni);
-        int made = 0, p = mt[rb].pos;
+        int made = 0, p = mt[rb].pos, twists = 0;
         while (made < 2 * N) {
             if (p >= kMtN) {
+                // each twisted key is kept, so the commit takes the state
+                // after the words drawn without twisting again
                 mt_twist_block(key);
+                for (int i = tid; i < kMtN; i += nthr) keys[(int64_t)twists * kMtN + i] = key[i];
+                ++twists;
                 p = 0;
             }
             const int take = min(kMtN - p, 2 * N - made);
@@ -1222,6 +1226,7 @@ sweep_prep_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
             const uint32_t w0 = wb[2 * i], w1 = wb[2 * i + 1];
             uni[i] = words_to_double(w0, w1);
         }
+        if (tid == 0) ctrl[kSwTwists] = twists;
     } else {
         const int ucount = u_cnt[rb];
         const int used0 = words ? u_used[rb] : 0;
@@ -1398,38 +1403,43 @@ sweep_field_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, 
 
 // One block of 1024 threads: the draw indices (a scan of the walk flags in
 // id order), the moves of X^{m+1}, the comparison with X^m and the next
-// candidates.
+// candidates.  The agents go in chunks of one per thread, so every load is
+// coalesced; a chunk's draw indices come from wave ballots and the waves'
+// totals in LDS.
 __global__ void __launch_bounds__(1024)
 sweep_motion_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, SweepX Xc, SweepX Xn,
                     const double* __restrict__ uni, int32_t* __restrict__ cand, int32_t* __restrict__ ctrl) {
 #pragma clang fp contract(off)
-    __shared__ int scan[1024];
+    __shared__ int wtot[2][16];
     __shared__ int changed;
     if (ctrl[kSwConverged] | ctrl[kSwSkip]) return;   // uniform over the block
     const FGRep& P = reps[rb];
     const int N = P.n_agents, a0 = P.agent_offset, tid = threadIdx.x, nthr = blockDim.x;
-    const int per = (N + nthr - 1) / nthr, lo = tid * per, hi = min(N, lo + per);
-    int mine = 0;
-    for (int k = lo; k < hi; ++k) mine += Xn.kind[k] == 0;
-    scan[tid] = mine;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
     if (tid == 0) changed = 0;
-    __syncthreads();
-    for (int off = 1; off < nthr; off <<= 1) {   // inclusive Hillis-Steele scan
-        const int add = tid >= off ? scan[tid - off] : 0;
-        __syncthreads();
-        scan[tid] += add;
-        __syncthreads();
-    }
-    int d = scan[tid] - mine;   // draws before agent lo
-    bool diff = false;
     auto bits_ne = [](double a, double b) { return __double_as_longlong(a) != __double_as_longlong(b); };
-    for (int k = lo; k < hi; ++k) {
+    bool diff = false;
+    int d_base = 0;   // draws before the chunk (uniform over the block)
+    for (int c0 = 0, par = 0; c0 < N; c0 += nthr, par ^= 1) {
+        const int k = c0 + tid;
+        const bool in = k < N;
+        const int kind = in ? Xn.kind[k] : 1;
+        const uint64_t m = __ballot(kind == 0);
+        if (lane == 0) wtot[par][wave] = (int)__popcll(m);
+        __syncthreads();   // wtot[par] is rewritten two chunks later, after this chunk's barrier
+        int d = d_base + (int)__popcll(m & ((1ull << lane) - 1ull)), tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int t = wtot[par][w];
+            if (w < wave) d += t;
+            tot += t;
+        }
+        d_base += tot;
+        if (!in) continue;
         const int g = a0 + k;
         double dth = Xn.dth[k];
-        if (Xn.kind[k] == 0) {
+        if (kind == 0) {
             const double uu = d < N ? uni[d] : NAN;
             dth = P.exp_theta_min + (P.exp_theta_max - P.exp_theta_min) * uu;
-            ++d;
         }
         Pose p{st.x[g], st.y[g], st.theta[g], st.v[g]};
         p.theta = prove_orientation(p.theta + dth);
@@ -1445,12 +1455,12 @@ sweep_motion_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
         diff = diff || bits_ne(p.x, Xc.x[k]) || bits_ne(p.y, Xc.y[k]) || bits_ne(p.theta, Xc.th[k]) ||
                bits_ne(p.v, Xc.v[k]) || bits_ne(Xn.w[k], Xc.w[k]) || bits_ne(Xn.u[k], Xc.u[k]) ||
                bits_ne(Xn.ip[k], Xc.ip[k]) || Xn.over[k] != Xc.over[k] || Xn.mode[k] != Xc.mode[k] ||
-               Xn.kind[k] != Xc.kind[k];
+               kind != Xc.kind[k];
     }
     if (diff) changed = 1;   // benign race: every writer stores 1
     __syncthreads();
-    if (tid == nthr - 1) {
-        ctrl[kSwDraws] = scan[nthr - 1];
+    if (tid == 0) {
+        ctrl[kSwDraws] = d_base;
         ctrl[kSwIters] += 1;
     }
     if (!changed) {
@@ -1462,11 +1472,13 @@ sweep_motion_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
 
 // Writes the converged sweep into the state (one block of 1024 threads):
 // poses, decision state, collected_r_before, the words used and status, and
-// the device generator advanced by the words drawn.
+// the device generator advanced by the words drawn (the key the prep
+// twisted to).
 __global__ void __launch_bounds__(1024)
 sweep_commit_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st, SweepX X,
                     int32_t* __restrict__ u_used, int32_t* __restrict__ status, int words_or_mt,
-                    abm_mt19937_state* __restrict__ mt, int32_t* __restrict__ ctrl) {
+                    abm_mt19937_state* __restrict__ mt, const uint32_t* __restrict__ keys,
+                    int32_t* __restrict__ ctrl) {
     __shared__ uint32_t key[kMtN];
     if (ctrl[kSwSkip]) return;
     if (!ctrl[kSwConverged]) {
@@ -1494,11 +1506,15 @@ sweep_commit_kernel(const FGRep* __restrict__ reps, int rb, abm_forage_state st,
             v = (pos - 1) / kMtN;
             pos -= kMtN * v;
         }
-        if (v > 0) {
+        // the prep kept every key it twisted, and c <= 2N words never need
+        // more; twisting here again is only the fallback
+        if (v > ctrl[kSwTwists]) {   // uniform over the block
             for (int i = tid; i < kMtN; i += nthr) key[i] = mt[rb].key[i];
             __syncthreads();
             for (int t = 0; t < v; ++t) mt_twist_block(key);
             for (int i = tid; i < kMtN; i += nthr) mt[rb].key[i] = key[i];
+        } else if (v > 0) {
+            for (int i = tid; i < kMtN; i += nthr) mt[rb].key[i] = keys[(int64_t)(v - 1) * kMtN + i];
         }
         if (tid == 0) mt[rb].pos = pos;
     }
@@ -1698,7 +1714,7 @@ __device__ int place_patches(const WRep& W, abm_patch_state ps, int b, int kind,
 // every (patch, agent) pair in parallel from the poses at the start of the
 // step; the loop over patches and their agents runs per agent in parallel,
 // with only each patch's depletion chain in id order (one thread per patch).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wreps, int phase,
                     abm_forage_state st, int32_t* __restrict__ pool_success, abm_patch_state ps,
                     int32_t* __restrict__ kill_rank, int32_t* __restrict__ pend,
@@ -1752,23 +1768,30 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
             s_pr[p] = ps.radius[s]; s_px[p] = ps.x[s]; s_py[p] = ps.y[s];
             s_pq[p] = ps.quality[s]; s_pl[p] = ps.left[s]; s_pid[p] = ps.id[s];
         }
-        // ---- membership: supcalc.distance(resc, agent) < resc.radius
-        const int total = np_ * C * 64;
-        for (int base = tid & ~63; base < total; base += nthr) {
-            const int idx = base + (tid & 63);
-            const int p = idx / (C * 64), j = idx - p * (C * 64);
-            bool in = false;
+        __syncthreads();
+        // ---- membership: supcalc.distance(resc, agent) < resc.radius.  A
+        // wave takes 64 agents, loads their centres once and tests them
+        // against every patch; the norm (a sqrt) only where both centre
+        // differences are within the radius (with a margin far above the
+        // rounding of the norm, so no pair inside is skipped)
+        for (int base = tid & ~63; base < C * 64; base += nthr) {
+            const int j = base + (tid & 63);
+            double cx = 0.0, cy = 0.0;
             if (j < N) {
-                const int64_t s = po + p;
-                const double pr = ps.radius[s];
-                const double pcx = ps.x[s] + pr, pcy = ps.y[s] + pr;
                 const int g = a0 + j;
                 const double ra = (double)st.radius[g];
-                const double dx = (st.x[g] + ra) - pcx, dy = (st.y[g] + ra) - pcy;
-                in = norm2(dx, dy) < pr;
+                cx = st.x[g] + ra;
+                cy = st.y[g] + ra;
             }
-            const uint64_t m = __ballot(in);
-            if ((tid & 63) == 0) member[p * C + (j >> 6)] = m;
+            for (int p = 0; p < np_; ++p) {
+                const double pr = s_pr[p];
+                const double dx = cx - (s_px[p] + pr), dy = cy - (s_py[p] + pr);
+                const double lim = pr * 1.0000001;
+                bool in = false;
+                if (j < N && fabs(dx) <= lim && fabs(dy) <= lim) in = norm2(dx, dy) < pr;
+                const uint64_t m = __ballot(in);
+                if ((tid & 63) == 0) member[p * C + (j >> 6)] = m;
+            }
         }
         __syncthreads();
         // ---- the patches' agent groups (sims.py:803-845).  The reference
@@ -2077,15 +2100,20 @@ constexpr int kForageMaxAgents = 65536;
 // Device bytes of the sweep buffers for replicas of up to M agents: two
 // tentative states (nine double and three int arrays each), the uniforms,
 // the candidates and the control words.
+// generator keys the prep may twist to while making 2M words
+int sweep_key_slots(int M) { return (2 * M + kMtN) / kMtN + 1; }
+
 size_t sweep_bytes(int M) {
     return 2 * (size_t)M * (9 * sizeof(double) + 3 * sizeof(int32_t)) + (size_t)M * sizeof(double) +
-           (size_t)M * sizeof(int32_t) + kSwWords * sizeof(int32_t) + 256;
+           (size_t)M * sizeof(int32_t) + sizeof(uint32_t) * kMtN * (size_t)sweep_key_slots(M) +
+           kSwWords * sizeof(int32_t) + 256;
 }
 
 struct SweepMem {
     SweepX X[2];
     double* uni;
     int32_t* cand;
+    uint32_t* keys;
     int32_t* ctrl;
 };
 
@@ -2103,6 +2131,7 @@ SweepMem sweep_carve(void* base, int M) {
         for (int32_t** a : i) { *a = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * (size_t)M; }
     }
     m.cand = reinterpret_cast<int32_t*>(p); p += sizeof(int32_t) * (size_t)M;
+    m.keys = reinterpret_cast<uint32_t*>(p); p += sizeof(uint32_t) * kMtN * (size_t)sweep_key_slots(M);
     m.ctrl = reinterpret_cast<int32_t*>(p);
     return m;
 }
@@ -2160,7 +2189,7 @@ int sweep_replica(abm_forage_batch* b, int rb, const abm_forage_state* st, const
     const SweepMem m = sweep_carve(b->sweep_mem, b->sweep_max_agents);
     const FGRep* reps = (const FGRep*)b->reps_dev;
     hipLaunchKernelGGL(sweep_prep_kernel, dim3(1), dim3(1024), 0, s, reps, rb, *st, uniforms, words, offset,
-                       count, used, status, mt, m.X[0], m.uni, m.cand, m.ctrl);
+                       count, used, status, mt, m.X[0], m.uni, m.keys, m.cand, m.ctrl);
     if (int rc = check_hip(hipGetLastError(), "sweep_prep_kernel launch")) return rc;
     int par = 0, sweeps = 0, round = 3;
     for (;;) {
@@ -2188,7 +2217,7 @@ int sweep_replica(abm_forage_batch* b, int rb, const abm_forage_state* st, const
         round = 2;
     }
     hipLaunchKernelGGL(sweep_commit_kernel, dim3(1), dim3(1024), 0, s, reps, rb, *st, m.X[par], used, status,
-                       (words || mt) ? 1 : 0, mt, m.ctrl);
+                       (words || mt) ? 1 : 0, mt, m.keys, m.ctrl);
     return check_hip(hipGetLastError(), "sweep_commit_kernel launch");
 }
 
@@ -2484,7 +2513,10 @@ int world_run(abm_forage_world* w, int32_t phase, const abm_forage_state* st, in
     abm_forage_batch* b = w->batch;
     const size_t lds = w->member_lds;
     if (lds > 65536) return set_error(ABM_ERR_UNSUPPORTED, "%s: %zu B of membership masks", who, lds);
-    hipLaunchKernelGGL(forage_world_kernel, dim3(b->n_replicas), dim3(256), lds, (hipStream_t)stream,
+    // a replica above 1024 agents takes a block of 1024 threads for the
+    // per-agent parts; smaller ones share a CU in blocks of 256
+    const int threads = b->max_agents > 1024 ? 1024 : 256;
+    hipLaunchKernelGGL(forage_world_kernel, dim3(b->n_replicas), dim3(threads), lds, (hipStream_t)stream,
                        (const FGRep*)b->reps_dev, (const WRep*)w->wreps_dev, (int)phase, *st, pool_success,
                        *ps, w->kill_rank, w->pend, w->pend_id, words, word_offset, word_count, words_used,
                        status, mt);

@@ -393,3 +393,54 @@ def test_world_membership_over_several_words(gpu):
         compare(sw, rw, world, 0, step)
         load_agents(sw, rw, world.agents, 0)
     assert kills > 0 and crowded == 3, (kills, crowded)   # a patch held agents of all three words
+
+
+def test_world_overlapping_caller_patches(gpu):
+    # patches a caller loads may overlap (the loop never places such): an
+    # agent in two groups is walked twice, in Group order, as in the
+    # reference's loop (sims.py:803-845); the kernel detects the shared
+    # agent and walks that replica's groups in one thread.  A second replica
+    # with separate patches takes the parallel walk in the same launch.
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    N, W, R = 40, 300, 256
+    wk = dict(N_resc=3, patch_radius=40, min_resc_perpatch=3, max_resc_perpatch=9,
+              min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False)
+    layouts = [[(1, (80, 80)), (2, (120, 80)), (3, (220, 220))],     # 1 and 2 overlap
+               [(1, (40, 40)), (2, (140, 80)), (3, (220, 220))]]     # apart
+    worlds, rngs, orngs = [], [], []
+    for b, layout in enumerate(layouts):
+        ags, p = make_agents(2200 + b, N, W, R, True, 0)
+        r0 = np.random.RandomState(2210 + b)
+        for a in ags:   # crowd the agents over the first two patches
+            a.position = np.array([r0.randint(100, 160), r0.randint(90, 130)], dtype=np.float64)
+        world = fo.ForageWorld(ags, W, W, 30, 0, 40, 3, 9, 0.5, 1.0)
+        world.patches = [fo.Patch(i, 40, pos, 5, 0.75) for i, pos in layout]
+        worlds.append(world)
+        rngs.append(np.random.RandomState(2220 + b))
+        orngs.append(np.random.RandomState(2220 + b))
+    cfg = forage_config(N, W, R, p)
+    allags = [a for w in worlds for a in w.agents]
+    sw = ForageSwarm([cfg, cfg], [a.position[0] for a in allags], [a.position[1] for a in allags],
+                     [a.orientation for a in allags], radius=10, rngs=rngs)
+    rw = ResourceWorld(sw, [WorldConfig(**wk)] * 2)
+    for b, world in enumerate(worlds):
+        load_agents(sw, rw, world.agents, b)
+        pt = world.patches
+        rw.set_patches(b, [q.id for q in pt], [q.position[0] for q in pt], [q.position[1] for q in pt],
+                       [q.radius for q in pt], [q.resc_units for q in pt], [q.resc_left for q in pt],
+                       [q.unit_per_timestep for q in pt])
+    shared = 0
+    for step in range(12):
+        print(f"  overlap step {step}", flush=True)
+        q1, q2 = worlds[0].patches[0], worlds[0].patches[1]
+        shared += sum(fo.distance(q1, a) < q1.radius and fo.distance(q2, a) < q2.radius
+                      for a in worlds[0].agents)
+        rw.step()
+        for b, world in enumerate(worlds):
+            np.random.set_state(orngs[b].get_state())
+            world.step()
+            orngs[b].set_state(np.random.get_state())
+            assert same_stream(rngs[b], orngs[b]), (step, b)
+            compare(sw, rw, world, b, (step, b))
+            load_agents(sw, rw, world.agents, b)
+    assert shared > 0
