@@ -126,6 +126,21 @@ __device__ inline float atan_pos_f32(float y, float x) {
     return y > x ? kHalfPiF - p : p;
 }
 
+// atan(z) for 0 <= z < 1 (the half-width angle atan(size / d) with d > size):
+// the same polynomial without the octant reduction.
+__device__ inline float atan_unit_f32(float z) {
+    const float s = z * z;
+    float p = -0.0046934f;
+    p = fmaf(p, s, 0.02425289f);
+    p = fmaf(p, s, -0.05948712f);
+    p = fmaf(p, s, 0.09914348f);
+    p = fmaf(p, s, -0.14019503f);
+    p = fmaf(p, s, 0.19969729f);
+    p = fmaf(p, s, -0.3333199f);
+    p = fmaf(p, s, 0.9999999f);
+    return p * z;
+}
+
 // Per-replica constants, all float32 except the few integers.
 struct RepConst {
     int N, R, W, a0, torus, fov_px0, fov_px1, phi_ok, limit;
@@ -191,16 +206,20 @@ __device__ inline PairF32 fast_pair(float u1x, float u1y, float fx, float fy, fl
         ok = (fabsf(ex) >= C.tguard) & (fabsf(ey) >= C.tguard);
         // object shifted down by W (object above), or focal shifted down by W
         // (object below); selects, no min / max (those would canonicalise)
+        // (x - 0 is x: the unshifted sides are the coordinates themselves, and
+        // the focal's shifted coordinates do not depend on the object)
         const bool wx = ex > 0.f, px = dx > 0.f, wy = ey > 0.f, py = dy > 0.f;
-        const float ox = (wx & px) ? C.fW : 0.f, fxs = (wx & !px) ? C.fW : 0.f;
-        const float oy = (wy & py) ? C.fH : 0.f, fys = (wy & !py) ? C.fH : 0.f;
-        dx = ((ojx - ox) - (fx - fxs)) + dr;
-        dy = ((ojy - oy) - (fy - fys)) + dr;
+        const float oxs = (wx & px) ? ojx - C.fW : ojx, fxs = (wx & !px) ? fx - C.fW : fx;
+        const float oys = (wy & py) ? ojy - C.fH : ojy, fys = (wy & !py) ? fy - C.fH : fy;
+        dx = (oxs - fxs) + dr;
+        dy = (oys - fys) + dr;
     }
     const float d2 = fmaf(dy, dy, dx * dx);
-    const float d = __builtin_amdgcn_sqrtf(d2);
     const float a = atan2_f32(fmaf(u1x, dy, -u1y * dx), fmaf(u1x, dx, u1y * dy));
-    const float q = atan_pos_f32(ojr, d);
+    // atan(size / d) from size / d = size * rsqrt(d^2) (v_rsq_f32, 1 ulp);
+    // a pair with d <= size (overlapping circles) goes to float64
+    const float z = ojr * __builtin_amdgcn_rsqf(d2);
+    const float q = atan_unit_f32(z);
     const float xb = (kPiF - a) * C.inv_step;       // bin coordinate of ca = -a
     // nearest bin; a tie (argmin takes the first index) lies inside the guard
     const float tr = rintf(xb);
@@ -211,8 +230,8 @@ __device__ inline PairF32 fast_pair(float u1x, float u1y, float fx, float fy, fl
     const float hg = 4e-6f * half + 1e-6f;
     // bitwise & (no short circuit): the compiler keeps the tests branch-free.
     // Outside the guard |a| < pi - guard, so t = rint(xb) lies in [0, R - 1].
-    const bool other = ok & (d2 > 1e-30f) & (fabsf(a) < kPiF - kAngleGuardF) & (hfrac >= hg) &
-                       (hfrac <= 1.0f - hg);
+    const bool other = ok & (d2 > 1e-30f) & (z < 1.0f) & (fabsf(a) < kPiF - kAngleGuardF) &
+                       (hfrac >= hg) & (hfrac <= 1.0f - hg);
     const bool tie_ok = fabsf(dfr) <= 0.5f - C.g_bins;
     const int t = (int)tr;
     const int h = (int)hfl;

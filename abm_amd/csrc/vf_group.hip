@@ -59,33 +59,34 @@ __device__ inline GroupWaveSmem group_wave_carve(char* p, int G, int maxN, int W
 // the guard bands the half-width is below R/4, so at most one of start < 0 and
 // end >= R holds and the wrap rules of vf_supcalc.py:122-129 reduce to two
 // pieces without clamps: [max(start, 0), min(end, R)) and the wrapped piece
-// [R + start, R) or [0, end - (R - 1)).  For every lane (drawn or not) the
-// piece starts lie in [0, R), so every lane issues the same LDS ORs with zero
-// masks where nothing is drawn; second words, wrapped pieces and pieces of 32
-// rays or more take wave-uniform branches.
+// [R + start, R) or [0, end - (R - 1)).  Only lanes with bits to set take
+// part in an LDS OR (lanes of the same word serialise on its address, and
+// idle lanes would add to that); second words, wrapped pieces and pieces of
+// 32 rays or more take wave-uniform branches.
 __device__ inline void raster_pred(uint32_t* F, int start, int end, int R, bool draw) {
-    const bool neg = start < 0, over = end >= R;
     const int p0 = start > 0 ? start : 0;
     int n0 = (end < R ? end : R) - p0;
-    const int q0 = neg ? R + start : 0;
-    int n1 = neg ? -start : (over ? end - (R - 1) : 0);
     n0 = draw ? n0 : 0;
-    n1 = draw ? n1 : 0;
-    const bool long0 = n0 > 31, long1 = n1 > 31;
+    const bool long0 = n0 > 31;
     uint32_t lo, hi;
     span_masks(p0 & 31, long0 ? 0 : n0, &lo, &hi);
-    atomicOr(&F[p0 >> 5], lo);
-    if (__ballot(hi != 0u)) atomicOr(&F[(p0 >> 5) + 1], hi);
-    if (__ballot(n1 > 0)) {
+    if (lo != 0u) atomicOr(&F[p0 >> 5], lo);
+    if (hi != 0u) atomicOr(&F[(p0 >> 5) + 1], hi);
+    // the wrapped piece, taken only when some lane has one
+    const bool neg = start < 0, over = end >= R;
+    if (__ballot(draw & (neg | over))) {
+        const int q0 = neg ? R + start : 0;
+        int n1 = neg ? -start : (over ? end - (R - 1) : 0);
+        n1 = draw ? n1 : 0;
+        const bool long1 = n1 > 31;
         uint32_t lo1, hi1;
         span_masks(q0 & 31, long1 ? 0 : n1, &lo1, &hi1);
-        atomicOr(&F[q0 >> 5], lo1);
-        if (__ballot(hi1 != 0u)) atomicOr(&F[(q0 >> 5) + 1], hi1);
-    }
-    if (__ballot(long0 || long1)) {
-        if (long0) or_range(F, p0, p0 + n0);
+        if (lo1 != 0u) atomicOr(&F[q0 >> 5], lo1);
+        if (hi1 != 0u) atomicOr(&F[(q0 >> 5) + 1], hi1);
         if (long1) or_range(F, q0, q0 + n1);
     }
+    if (__ballot(long0))
+        if (long0) or_range(F, p0, p0 + n0);
 }
 
 template <int G, int RPL, bool KH>
