@@ -145,6 +145,7 @@ _SIGS = {
                                             C.POINTER(C.c_int64), _P]),
     "abm_vf_f64_near_ties": (C.c_int, [_P, _P, _P]),
     "abm_vf_set_launch": (C.c_int, [_P, _I32]),
+    "abm_vf_last_kernels": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "abm_vf_set_grouping": (C.c_int, [_P, _I32]),
     "abm_vf_autotune": (C.c_int, [_P, C.POINTER(VFStateF32), _I32, C.POINTER(_I32),
                                   C.POINTER(C.c_float), _P]),

@@ -278,6 +278,7 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
         d.table_offset = table_off[r.n_rays];
         d.boundary = r.boundary;
         d.step = linspace_step(r.n_rays);
+        d.cull_c = (1.0 / std::tan(kTwoPi / r.n_rays)) * (1.0 + 2e-4);
         d.fov_px0 = linspace_argmin(r.fov0, r.n_rays, d.step);
         d.fov_px1 = linspace_argmin(r.fov1, r.n_rays, d.step);
         d.phi_ok = arange_len(r.n_rays) == r.n_rays ? 1 : 0;
@@ -338,6 +339,12 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
             bt->small_max_agents = std::max(bt->small_max_agents, host[b].n_agents);
             bt->small_max_rays = std::max(bt->small_max_rays, host[b].n_rays);
         }
+    }
+    // the push kernel's LDS: its largest replica (all, or the large size class)
+    for (int b = 0; b < n; ++b) {
+        const size_t push = lds_bytes_push(host[b].n_agents, host[b].n_rays);
+        bt->push_lds_all = std::max(bt->push_lds_all, push);
+        if (host[b].n_agents > kBigClass) bt->push_lds_big = std::max(bt->push_lds_big, push);
     }
     if (!rc) rc = check_hip(hipMalloc(&bt->reps_dev, sizeof(VFRep) * n), "hipMalloc(replicas)");
     if (!rc) rc = check_hip(hipMalloc(&bt->tab_cos, sizeof(double) * 3 * ray_total), "hipMalloc(tables)");
@@ -472,10 +479,16 @@ int abm_vf_f64_near_ties(abm_vf_batch* b, int64_t* near_ties, void* stream) {
     return ABM_OK;
 }
 
+int abm_vf_last_kernels(const abm_vf_batch* b, uint32_t* mask) {
+    if (!b || !mask) return set_error(ABM_ERR_INVALID, "abm_vf_last_kernels: NULL argument");
+    *mask = b->last_kernels;
+    return ABM_OK;
+}
+
 int abm_vf_set_launch(abm_vf_batch* b, int32_t waves) {
     if (!b) return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: NULL batch");
-    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 16)
-        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2, 4 or 16");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 16 && waves != 32)
+        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2, 4, 16 or 32");
     b->waves_per_group = waves;
     b->group_g = waves > 0 ? -1 : 0;
     return ABM_OK;
@@ -505,11 +518,13 @@ int abm_vf_autotune(abm_vf_batch* b, const abm_vf_state_f32* st, int32_t trial_s
     if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) { (void)hipEventDestroy(e0); return rc; }
     // candidates: waves per replica of the per-replica kernel (> 0), or minus the
     // replicas per wave of the grouped kernel
-    // (16: the 16-wave kernel for every replica; 0: the size classes, the
-    // per-replica kernel up to kBigClass agents and the 16-wave kernel above)
+    // (16: the 16-wave kernel for every replica; 32: the push kernel for
+    // every replica; 0: the size classes, the per-replica kernel up to
+    // kBigClass agents and the push or 16-wave kernel above)
     std::vector<int> cand{1, 2, 4};
     if (b->max_agents <= 2048) cand.push_back(16);
-    if (b->n_big_class > 0 && b->n_big_class < b->n_replicas) cand.push_back(0);
+    if (push_fits(b, true)) cand.push_back(32);
+    if (b->n_big_class > 0) cand.push_back(0);
     if (b->group_ok) { cand.push_back(-1); cand.push_back(-2); cand.push_back(-4); }
     int best = 1;
     float best_ms = 1e30f;

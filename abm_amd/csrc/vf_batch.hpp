@@ -41,6 +41,7 @@ struct VFRep {
     double gam, v0, alp0, alp1, bet0, bet1;
     double max_vel, max_th;
     double step;            // linspace(-pi, pi, R) step
+    double cull_c;          // visibility cut-off d > radius * cull_c: cot(2 pi / R) (1 + 2e-4)
 };
 
 // float32 ray-table entry used by the fused kernel, one per ray.
@@ -61,6 +62,24 @@ struct GroupPlan {
 // batch: the 16-wave (one workgroup per replica) kernel updates them, the
 // per-replica kernel the rest.
 constexpr int kBigClass = 128;
+
+// The push kernel (one workgroup of 8 waves per replica): rays per lane of
+// its summing wave for a launch whose largest R is r (the one-wave
+// per-replica kernel's rule), and its LDS for a replica of n agents and r
+// rays: start and new poses, speeds and parameters, heading vectors, the
+// pushing waves' queues, and every agent's field (one spare word per row).
+constexpr int kPushWaves = 8;
+constexpr int kPushQueue = 128;   // each pushing wave's ring of queued pairs
+inline int push_rpl(int r) {
+    const int need = (r + 63) / 64;
+    return need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 16 ? 16 : 32));
+}
+inline size_t lds_bytes_push(int n, int r) {
+    const int w1 = ((r + 31) >> 5) + 1;
+    const size_t b = sizeof(float) * 4 * 3 * (size_t)n + sizeof(float) * 2 * (size_t)n +
+                     sizeof(uint32_t) * kPushQueue * kPushWaves + sizeof(uint32_t) * (size_t)n * w1;
+    return (b + 15) & ~(size_t)15;
+}
 
 // Largest replica the grouped kernel takes, and the largest R.
 constexpr int kGroupMaxAgents = 256;
@@ -93,6 +112,9 @@ struct abm_vf_batch {
     int32_t* order_dev = nullptr;   // replica launch order: the large size class first, each
                                     // class by descending work
     int32_t n_big_class = 0;        // replicas above kBigClass agents (first in order_dev)
+    uint32_t last_kernels = 0;      // ABM_KERNEL_* bits of the last step's launches
+    size_t push_lds_all = 0;        // push kernel LDS of the largest replica (bytes)
+    size_t push_lds_big = 0;        // ... of the largest in the large size class
     int32_t small_max_agents = 0, small_max_rays = 0;   // of the other replicas
     int32_t big_max_agents = 0, big_max_rays = 0;       // of the large class
     float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
@@ -112,6 +134,9 @@ int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bi
                        double* diag, uint32_t flags, hipStream_t s);
 int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
                        float* diag, uint32_t flags, hipStream_t s);
+// Whether the push kernel holds every replica (all) or those of the large
+// size class.
+bool push_fits(const abm_vf_batch* b, bool all);
 int launch_vf_group_f32(abm_vf_batch* b, int G, const abm_vf_state_f32* st, uint32_t* bits,
                         float* diag, uint32_t flags, hipStream_t s);
 int group_rpl(int G, int R);

@@ -729,6 +729,7 @@ int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, flo
                            (const float4*)b->wide_out, *st, h.agent_offset, h.n_agents);
         if (int rc = check_hip(hipGetLastError(), "copy_back_kernel launch")) return rc;
     }
+    b->last_kernels |= ABM_KERNEL_WIDE;
     return ABM_OK;
 }
 
@@ -989,6 +990,7 @@ int launch_big(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, floa
     const float cull_c = (float)((1.0 / std::tan(kTwoPi / maxR)) * (1.0 + 2e-4));
     hipLaunchKernelGGL(kern, dim3(n_rep), dim3(kBigThreads), lds, s, b->reps_dev, order, b->tab32, *st,
                        bits, diag, flags, b->counters, cull_c, maxN, maxR);
+    b->last_kernels |= ABM_KERNEL_BIG;
     return check_hip(hipGetLastError(), "vf_step_f32_big_kernel launch");
 }
 
@@ -996,6 +998,344 @@ int launch_big_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, 
                    uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, int maxN, int maxR) {
     if (maxR <= kBigThreads) return launch_big<1>(b, st, bits, diag, flags, s, order, n_rep, maxN, maxR);
     return launch_big<2>(b, st, bits, diag, flags, s, order, n_rep, maxN, maxR);
+}
+
+// ---------------------------------------------------------------------------
+// Reference (id) order with the pairs off the id-order chain (one workgroup
+// of 512 threads per replica, every agent's field in LDS).  Agent m's field
+// is the OR of its pairs with every other agent j, seen at j's new pose for
+// j < m and at its start pose for j > m, and m itself at its start pose.
+// Step k of the chain (one barrier each):
+//   * wave 0 sums agent k's field, moves agent k and decides the pair
+//     (k + 1, k) at k's new pose;
+//   * waves 1..7 decide the pairs (m, k - 1) for m > k with agent k - 1's
+//     new pose.
+// The start-pose pairs (m, j > m) are decided before the chain by every
+// wave.  Pairs beyond the visibility cut-off are dropped by a cheap float32
+// test (as in the 16-wave kernel) and the rest queued, so that the exact
+// decisions run on full waves.
+// So only one pair, agent k's sums and its move stay on the chain.  Wave 0
+// sums and moves exactly as the one-wave per-replica kernel does (same ray
+// spans, table order and reductions), and the decisions are exact, so the
+// two kernels give identical results bit for bit.  Wave 0 keeps its rays'
+// table entries in registers (up to 128), which two waves per SIMD allow;
+// from LDS, the loads of the sums waited on the other waves' LDS traffic.
+constexpr int kPushThreads = 64 * kPushWaves;
+
+// One pair of the push kernel: focal f (start pose, heading vector u), object
+// o; the float32 decision with the float64 one inside the guard bands (and
+// every pair in verify mode), rastered into the focal's field Fm.
+__device__ inline void push_pair(uint32_t* Fm, float2 u, float4 f, float4 o, const VFRep* rp,
+                                 const RepConst& C, bool count, bool verify, unsigned long long& n_pairs,
+                                 unsigned long long& n_recheck, unsigned long long& n_disagree) {
+    if (o.x == f.x && o.y == f.y) return;   // coincident (vf_supcalc.py:57)
+    PairF32 pr = fast_pair(u.x, u.y, f.x, f.y, f.z, o.x, o.y, o.z, C);
+    if (count) ++n_pairs;
+    if (pr.tie && tie_in_place(pr, f.x, f.y, f.w, f.z, o.x, o.y, o.z, rp, C))   // rare
+        if (count) ++n_recheck;
+    if (!pr.ok || verify) {
+        int es = 0, ee = 0, ed = 0;
+        exact_pair(f.x, f.y, f.w, f.z, o.x, o.y, o.z, rp, &es, &ee, &ed);
+        if (verify && pr.ok && (es != pr.start || ee != pr.end || ed != (int)pr.drawn)) ++n_disagree;
+        if (!pr.ok) {
+            if (count) ++n_recheck;
+            pr.start = es; pr.end = ee; pr.drawn = ed != 0;
+        }
+    }
+    if (pr.drawn) raster(Fm, pr.start, pr.end, C.R);
+}
+
+// Whether a pair lies beyond the visibility cut-off d > size cot(2 pi / R)
+// (floor(proj_size / 2) = 0: nothing drawn), in float32 with the periodic
+// image's guard band, as the 16-wave kernel's cheap pass.  A culled pair is
+// counted as decided; in verify mode its float64 raster must be empty.
+__device__ inline bool push_culled(float4 f, float4 o, const RepConst& C, float cull_c) {
+    float dx = (o.x - f.x) + (o.z - f.z), dy = (o.y - f.y) + (o.z - f.z);
+    bool near_wrap = false;
+    if (C.torus) {
+        const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
+        near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
+        if (ex > 0.f) dx = dx > 0.f ? dx - C.fW : dx + C.fW;
+        if (ey > 0.f) dy = dy > 0.f ? dy - C.fH : dy + C.fH;
+    }
+    const float lim = o.z * cull_c;
+    return !near_wrap && fmaf(dy, dy, dx * dx) > lim * lim;
+}
+
+__device__ inline void push_culled_check(float4 f, float4 o, const VFRep* rp, bool count, bool verify,
+                                         unsigned long long& n_pairs, unsigned long long& n_disagree) {
+    if (count) ++n_pairs;
+    if (verify) {
+        int es = 0, ee = 0, ed = 0;
+        exact_pair(f.x, f.y, f.w, f.z, o.x, o.y, o.z, rp, &es, &ee, &ed);
+        if (ed && es != ee) ++n_disagree;
+    }
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(kPushThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+vf_step_f32_push_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order,
+                        const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
+                        uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
+                        unsigned long long* __restrict__ counters) {
+    extern __shared__ __align__(16) double smem_d[];
+    const VFRep* rp = reps + order[blockIdx.x];
+    const RepConst C = rep_const(rp);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int W1 = C.W + 1;
+    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    // laid out for this replica (lds_bytes_push(N, R) bytes)
+    char* p = reinterpret_cast<char*>(smem_d);
+    float4* P0 = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)C.N;    // start poses
+    float4* P1 = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)C.N;    // new poses
+    float4* dyn = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)C.N;   // v, ALP0, BET0, V0
+    float2* U = reinterpret_cast<float2*>(p); p += sizeof(float2) * (size_t)C.N;     // (cos, -sin) of start headings
+    uint32_t* queue = reinterpret_cast<uint32_t*>(p); p += sizeof(uint32_t) * kPushQueue * kPushWaves;
+    uint32_t* F = reinterpret_cast<uint32_t*>(p);                                     // [N][W + 1] fields
+    const float cull_c = (float)rp->cull_c;
+    uint32_t* wq = queue + wave * kPushQueue;   // this wave's ring of queued pairs
+    const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
+    for (int i = tid; i < C.N; i += kPushThreads) {
+        const int g = C.a0 + i;
+        const float4 q = make_float4(st.x[g], st.y[g], (float)st.radius[g], st.theta[g]);
+        P0[i] = q;
+        dyn[i] = make_float4(st.v[g], st.alp0 ? st.alp0[g] : a0d, st.bet0 ? st.bet0[g] : b0d,
+                             st.v0 ? st.v0[g] : v0d);
+        float sth, cth;
+        sincos_f32(q.w, &sth, &cth);
+        U[i] = make_float2(cth, -sth);
+    }
+    // wave 0's lane l owns rays [l RPL, (l + 1) RPL) and their table entries
+    RayTab32 T[RPL];
+    if (wave == 0) {
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = lane * RPL + i;
+            if (r < C.R) T[i] = tab32[rp->table_offset + r];
+            else T[i].wcos = T[i].wsin = T[i].cos = T[i].sin = 0.f;
+        }
+    }
+    for (int w = tid; w < C.N * W1; w += kPushThreads) F[w] = 0u;
+    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    __syncthreads();
+    // ---- the start-pose pairs (m, j > m), wave w taking the rows m = w mod 8;
+    // queue entries m | j << 16
+    {
+        int head = 0, tail = 0;   // wave-uniform ring positions
+        for (int m = wave; m < C.N; m += kPushWaves) {
+            const float4 f = P0[m];
+            for (int jb = m + 1; jb < C.N; jb += 64) {
+                const int j = jb + lane;
+                bool keep = false;
+                if (j < C.N) {
+                    const float4 ob = P0[j];
+                    if (!(ob.x == f.x && ob.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
+                        keep = !push_culled(f, ob, C, cull_c);
+                        if (!keep) push_culled_check(f, ob, rp, count, verify, n_pairs, n_disagree);
+                    }
+                }
+                const uint64_t mk = __ballot(keep);
+                if (keep)
+                    wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kPushQueue - 1)] =
+                        (uint32_t)m | ((uint32_t)j << 16);
+                tail += (int)__popcll(mk);
+                wave_lds_sync();
+                if (tail - head >= 64) {
+                    const uint32_t e = wq[(head + lane) & (kPushQueue - 1)];
+                    const int mm = (int)(e & 0xffffu), jj = (int)(e >> 16);
+                    push_pair(F + (size_t)mm * W1, U[mm], P0[mm], P0[jj], rp, C, count, verify, n_pairs,
+                              n_recheck, n_disagree);
+                    head += 64;
+                    wave_lds_sync();
+                }
+            }
+        }
+        if (lane < tail - head) {
+            const uint32_t e = wq[(head + lane) & (kPushQueue - 1)];
+            const int mm = (int)(e & 0xffffu), jj = (int)(e >> 16);
+            push_pair(F + (size_t)mm * W1, U[mm], P0[mm], P0[jj], rp, C, count, verify, n_pairs, n_recheck,
+                      n_disagree);
+        }
+    }
+    __syncthreads();
+    const int rs = lane * RPL;   // wave 0: this lane's rays
+    const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
+    if (wave == 0) __builtin_amdgcn_s_setprio(3);   // the chain's wave issues first
+    for (int k = 0; k < C.N; ++k) {
+        if (wave == 0) {
+            // ---- agent k's sums, as the one-wave per-replica kernel
+            const uint32_t* Fk = F + (size_t)k * W1;
+            const bool case1 = (Fk[0] & 1u) == 1u && field_bit(Fk, C.R - 1) == 0u;
+            float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
+            {
+                const int w0 = rs >> 5;
+                const uint32_t lo = w0 < C.W ? Fk[w0] : 0u;
+                const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? Fk[w0 + 1] : 0u;
+                const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
+                const uint32_t m = low_mask(rn);
+                const uint32_t cur = (uint32_t)v64 & m;
+                uint32_t spike = 0u;
+                if (rn > 0) {
+                    if (case1) {
+                        const uint32_t prev = field_bit(Fk, rs == 0 ? C.R - 1 : rs - 1);
+                        spike = (cur ^ ((cur << 1) | prev)) & m;
+                    } else {
+                        const int nx = rs + rn;
+                        const uint32_t next = field_bit(Fk, nx >= C.R ? 0 : nx);
+                        spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) {
+                    const bool vb = (cur >> i) & 1u;
+                    const bool sb = (spike >> i) & 1u;
+                    bc += vb ? T[i].wcos : 0.f;
+                    bs += vb ? T[i].wsin : 0.f;
+                    ec += sb ? T[i].cos : 0.f;
+                    es += sb ? T[i].sin : 0.f;
+                }
+            }
+            const float4 fp = P0[k];
+            const float4 dk = dyn[k];
+            const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
+            const float alp0 = dk.y, bet0 = dk.z, v0 = dk.w;
+            double Sv, Sp, Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
+            if (!want_diag) {
+                Sv = wave_total(alp0 * (-bc) + (alp0 * C.alp1) * ec);
+                Sp = wave_total(bet0 * (-bs) + (bet0 * C.bet1) * es);
+            } else {
+                Bc = wave_total(bc); Bs = wave_total(bs);
+                Ec = wave_total(ec); Es = wave_total(es);
+                Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+                Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+            }
+            if (bits_out) {
+                uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
+                for (int w = lane; w < C.W; w += 64) dst[w] = Fk[w];
+            }
+            // ---- agent k's move, then the pair (k + 1, k) at its new pose
+            float vel, o, dv, dpsi;
+            steer(C, fth, dk.x, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
+            float so, co;
+            sincos_f32(o, &so, &co);
+            float nx, ny;
+            euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
+            const float4 moved = make_float4(nx, ny, fr, o);
+            if (lane == 0) {
+                const int gk = C.a0 + k;
+                P1[k] = moved;
+                st.v[gk] = vel;
+                if (want_diag) {
+                    float* dd = diag + 6 * (int64_t)gk;
+                    dd[0] = dv; dd[1] = dpsi;
+                    dd[2] = C.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
+                    dd[3] = C.phi_ok ? (float)((double)(alp0 * C.alp1) * Ec) : 0.f;
+                    dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
+                    dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
+                }
+                if (k + 1 < C.N) {
+                    const float4 f1 = P0[k + 1];
+                    if (push_culled(f1, moved, C, cull_c))
+                        push_culled_check(f1, moved, rp, count, verify, n_pairs, n_disagree);
+                    else
+                        push_pair(F + (size_t)(k + 1) * W1, U[k + 1], f1, moved, rp, C, count, verify,
+                                  n_pairs, n_recheck, n_disagree);
+                }
+            }
+        } else {
+            // ---- agent k - 1's new pose into the fields of agents k + 1 ..:
+            // each wave tests its focal agents against the cut-off 64 at a
+            // time and queues the rest in a ring of 128 in LDS, deciding 64
+            // at a time
+            if (k >= 1) {
+                const float4 o = P1[k - 1];
+                int head = 0, tail = 0;   // wave-uniform ring positions
+                for (int mb = k + 1 + (wave - 1) * 64; mb < C.N; mb += kPushThreads - 64) {
+                    const int m = mb + lane;
+                    bool keep = false;
+                    if (m < C.N) {
+                        const float4 f = P0[m];
+                        if (!(o.x == f.x && o.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
+                            keep = !push_culled(f, o, C, cull_c);
+                            if (!keep) push_culled_check(f, o, rp, count, verify, n_pairs, n_disagree);
+                        }
+                    }
+                    const uint64_t mk = __ballot(keep);
+                    if (keep) wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kPushQueue - 1)] = (uint32_t)m;
+                    tail += (int)__popcll(mk);
+                    wave_lds_sync();
+                    if (tail - head >= 64) {
+                        const int mm = (int)wq[(head + lane) & (kPushQueue - 1)];
+                        push_pair(F + (size_t)mm * W1, U[mm], P0[mm], o, rp, C, count, verify, n_pairs,
+                                  n_recheck, n_disagree);
+                        head += 64;
+                        wave_lds_sync();
+                    }
+                }
+                if (lane < tail - head) {
+                    const int mm = (int)wq[(head + lane) & (kPushQueue - 1)];
+                    push_pair(F + (size_t)mm * W1, U[mm], P0[mm], o, rp, C, count, verify, n_pairs, n_recheck,
+                              n_disagree);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < C.N; i += kPushThreads) {
+        const int g = C.a0 + i;
+        const float4 q = P1[i];
+        st.x[g] = q.x;
+        st.y[g] = q.y;
+        st.theta[g] = q.w;
+    }
+    if (count) {
+        if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+    }
+}
+
+// Whether the push kernel holds every replica (all) or the large size class.
+bool push_fits_in(const abm_vf_batch* b, bool all) {
+    const size_t lds = all ? b->push_lds_all : b->push_lds_big;
+    return lds > 0 && b->max_rays <= 64 * 32 && lds <= (size_t)b->dev_max_lds &&
+           (all ? b->max_agents : b->big_max_agents) <= kWideThreshold;
+}
+
+template <int RPL>
+int launch_push(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+                uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, size_t lds) {
+    auto kern = vf_step_f32_push_kernel<RPL>;
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    hipLaunchKernelGGL(kern, dim3(n_rep), dim3(kPushThreads), lds, s, b->reps_dev, order, b->tab32, *st,
+                       bits, diag, flags, b->counters);
+    b->last_kernels |= ABM_KERNEL_PUSH;
+    return check_hip(hipGetLastError(), "vf_step_f32_push_kernel launch");
+}
+
+// The push kernel over order[0 .. n_rep): every replica (all) or the large
+// size class.  Rays per lane of wave 0 follow the one-wave per-replica
+// kernel's rule for the largest R of the launch.
+int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
+                    uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, bool all) {
+    const size_t lds = all ? b->push_lds_all : b->push_lds_big;
+    if (!push_fits_in(b, all))
+        return set_error(ABM_ERR_UNSUPPORTED,
+                         "abm_vf_step_f32: the push kernel holds no replica of this batch (%zu B of LDS "
+                         "for its fields, device max %d; up to %d agents)", lds, b->dev_max_lds,
+                         kWideThreshold);
+    const int rpl = push_rpl(all ? b->max_rays : b->big_max_rays);
+    if (rpl == 4) return launch_push<4>(b, st, bits, diag, flags, s, order, n_rep, lds);
+    if (rpl == 8) return launch_push<8>(b, st, bits, diag, flags, s, order, n_rep, lds);
+    if (rpl == 16) return launch_push<16>(b, st, bits, diag, flags, s, order, n_rep, lds);
+    return launch_push<32>(b, st, bits, diag, flags, s, order, n_rep, lds);
 }
 
 size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
@@ -1040,10 +1380,13 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
     if (grid > n_rep) grid = n_rep;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NW), lds, s, b->reps_dev, order,
                        n_rep, b->tab32, *st, bits, diag, flags, b->counters, maxN);
+    b->last_kernels |= ABM_KERNEL_PER_REPLICA;
     return check_hip(hipGetLastError(), "vf_step_f32_kernel launch");
 }
 
 }  // namespace
+
+bool push_fits(const abm_vf_batch* b, bool all) { return push_fits_in(b, all); }
 
 // Replicas per wave of the grouped kernel for this batch, or 0 for the
 // per-replica kernel.  Automatic choice (measured on MI355X): batches with at
@@ -1088,6 +1431,7 @@ static int launch_per_replica(abm_vf_batch* b, const abm_vf_state_f32* st, uint3
 
 int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                        uint32_t flags, hipStream_t s) {
+    b->last_kernels = 0;
     if (b->max_rays > 2048)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: n_rays up to 2048 (got %d); use abm_vf_step_f64", b->max_rays);
@@ -1118,17 +1462,24 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
                          "grouped kernel only, which this batch does not take (one R up to %d rays, "
                          "replicas of at most %d agents)", kGroupMaxRays, kGroupMaxAgents);
     const int nw = b->waves_per_group;
+    if (nw == 32)   // the push kernel for every replica
+        return launch_push_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, true);
     if (nw == 16)   // the 16-wave kernel for every replica
         return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
                               b->max_rays);
     const int nb = b->n_big_class;
-    if (nw == 0 && nb > 0 && nb < b->n_replicas) {
+    const bool push = nb > 0 && push_fits(b, false);
+    if (nw == 0 && nb > 0 && (push || nb < b->n_replicas)) {
         // size classes: the small replicas on the per-replica kernel, then
-        // the large ones on the 16-wave kernel (their id-order chains are
-        // the step's critical path)
-        if (int rc = launch_per_replica(b, st, bits, diag, flags, s, 0, b->order_dev + nb,
-                                        b->n_replicas - nb, b->small_max_agents, b->small_max_rays, true))
-            return rc;
+        // the large ones, whose id-order chains are the step's critical
+        // path, on the push kernel (pairs off the chain) where their fields
+        // fit in LDS, else on the 16-wave kernel
+        if (nb < b->n_replicas) {
+            if (int rc = launch_per_replica(b, st, bits, diag, flags, s, 0, b->order_dev + nb,
+                                            b->n_replicas - nb, b->small_max_agents, b->small_max_rays, true))
+                return rc;
+        }
+        if (push) return launch_push_any(b, st, bits, diag, flags, s, b->order_dev, nb, false);
         return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, nb, b->big_max_agents,
                               b->big_max_rays);
     }

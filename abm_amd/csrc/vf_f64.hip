@@ -210,6 +210,7 @@ int launch_vf_step_f64(abm_vf_batch* b, const abm_vf_state_f64* st, uint32_t* bi
     hipLaunchKernelGGL(vf_step_f64_kernel, dim3(b->n_replicas), dim3(nthr), lds, s,
                        b->reps_dev, b->tab_cos, b->tab_sin, b->tab_dphi, *st, bits, diag, flags,
                        (unsigned long long*)b->counters);
+    b->last_kernels = ABM_KERNEL_F64;
     return check_hip(hipGetLastError(), "vf_step_f64_kernel launch");
 }
 

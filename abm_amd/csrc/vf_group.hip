@@ -347,6 +347,7 @@ int launch_group_shape(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
     if (grid > (long)occ * b->n_cu) grid = (long)occ * b->n_cu;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * nwg), lds, s, b->reps_dev, gp.groups,
                        gp.n_groups, gp.nib, *st, bits, diag, flags, b->counters, b->max_agents, W);
+    b->last_kernels |= ABM_KERNEL_GROUPED;
     return check_hip(hipGetLastError(), "vf_group_kernel launch");
 }
 

@@ -226,9 +226,18 @@ class VFSwarm:
     def set_launch(self, waves_per_group: int):
         """Per-replica fused kernel with this many waves per replica (1, 2, 4),
         the 16-wave kernel (one workgroup per replica) for every replica (16),
-        or the automatic choice (0: replicas above 128 agents on the 16-wave
-        kernel, the rest on the per-replica kernel)."""
+        the push kernel (one workgroup per replica, pairs off the id-order
+        chain) for every replica (32), or the automatic choice (0: replicas
+        above 128 agents on the push kernel where their fields fit in LDS,
+        else the 16-wave kernel, the rest on the per-replica kernel)."""
         L.check(self.batch.lib.abm_vf_set_launch(self.batch.handle, waves_per_group))
+
+    def last_kernels(self):
+        """Names of the kernels the last step launched."""
+        m = C.c_uint32()
+        L.check(self.batch.lib.abm_vf_last_kernels(self.batch.handle, C.byref(m)))
+        names = ("grouped", "per_replica", "wide", "big", "push", "f64")
+        return {n for i, n in enumerate(names) if m.value >> i & 1}
 
     def set_grouping(self, replicas_per_wave: int):
         """Grouped fused kernel with 1, 2 or 4 replicas per wave; -1 selects the

@@ -167,15 +167,29 @@ int abm_vf_f64_near_ties(abm_vf_batch* batch, int64_t* near_ties, void* stream);
  * 1, 2 or 4 waves cooperating on one replica, and the grouped kernel, with
  * 1, 2 or 4 replicas advanced in lockstep by one wave (for batches with one
  * n_rays <= 1024 and at most 256 agents per replica; replicas that differ in
- * more than N and the ALP0/BET0/V0 defaults never share a wave).  A third,
- * the 16-wave kernel, gives one workgroup of 1024 threads to each replica
- * (every pose in LDS); it runs single flocks above 2048 agents in the
- * reference order and, by default, the large size class of a ragged batch
- * (replicas above 128 agents) next to the per-replica kernel for the rest.
- * abm_vf_set_launch selects the per-replica kernel with waves_per_group
- * waves per replica (1, 2, 4), the 16-wave kernel for every replica (16),
- * or the automatic choice (0: size classes when the batch has both). */
+ * more than N and the ALP0/BET0/V0 defaults never share a wave).  Two more
+ * give one workgroup of 1024 threads to each replica: the 16-wave kernel
+ * (every pose in LDS, the pairs of each agent decided in its turn), which
+ * runs single flocks above 2048 agents in the reference order, and the push
+ * kernel (every agent's field in LDS, the pairs decided off the id-order
+ * chain as soon as their object has moved), which by default runs the large
+ * size class of a batch (replicas above 128 agents) when their fields fit,
+ * next to the per-replica kernel for the rest.  abm_vf_set_launch selects
+ * the per-replica kernel with waves_per_group waves per replica (1, 2, 4),
+ * the 16-wave kernel for every replica (16), the push kernel for every
+ * replica (32; ABM_ERR_UNSUPPORTED at step time when the fields do not fit
+ * in LDS), or the automatic choice (0). */
 int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
+/* The kernels the batch's last step launched (ABM_KERNEL_* bits), for
+ * callers that log or check the launch choice.  No counterpart in the
+ * reference. */
+#define ABM_KERNEL_GROUPED 1u
+#define ABM_KERNEL_PER_REPLICA 2u
+#define ABM_KERNEL_WIDE 4u
+#define ABM_KERNEL_BIG 8u
+#define ABM_KERNEL_PUSH 16u
+#define ABM_KERNEL_F64 32u
+int abm_vf_last_kernels(const abm_vf_batch* batch, uint32_t* mask);
 /* replicas_per_wave: 1, 2 or 4 = grouped kernel with that many replicas per
  * wave (ABM_ERR_UNSUPPORTED when the batch does not qualify), -1 = per-replica
  * kernel, 0 = automatic choice. */
