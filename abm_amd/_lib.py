@@ -96,6 +96,7 @@ class ForageWorldParams(C.Structure):
         ("n_patches", C.c_int32), ("patch_radius", C.c_int32), ("min_units", C.c_int32),
         ("max_units", C.c_int32), ("pooling_time", C.c_int32), ("regenerate", C.c_int32),
         ("teleport_exploit", C.c_int32), ("allow_border_overlap", C.c_int32),
+        ("collide_agents", C.c_int32), ("ghost_mode", C.c_int32),
         ("consumption", C.c_double), ("min_quality", C.c_double), ("max_quality", C.c_double),
     ]
 
@@ -106,7 +107,7 @@ class PatchState(C.Structure):
 
 
 WORLD_STEP, WORLD_RESUME, WORLD_FILL = 0, 1, 2                       # ABM_WORLD_* phases
-WORLD_DONE, WORLD_NEED_WORDS, WORLD_TIMEOUT, WORLD_BAD_COUNT = 0, 1, 2, 4   # statuses
+WORLD_DONE, WORLD_NEED_WORDS, WORLD_TIMEOUT, WORLD_BAD_COUNT, WORLD_CROWDED = 0, 1, 2, 4, 5   # statuses
 FORAGE_PATH_AUTO, FORAGE_PATH_SWEEP = 0, 1                             # ABM_FORAGE_PATH_*
 
 

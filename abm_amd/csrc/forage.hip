@@ -1531,7 +1531,7 @@ constexpr int kPendNone = 0, kPendKill = 1, kPendFill = 2;
 
 // Per-replica world constants on the device.
 struct WRep {
-    int32_t n_patches, patch_radius, pooling_time, regenerate, teleport, pad_;
+    int32_t n_patches, patch_radius, pooling_time, regenerate, teleport, collide, ghost, pad_;
     int64_t patch_offset;
     int64_t x_lo, x_hi, y_lo, y_hi, units_lo, units_hi;   // randint bounds [lo, hi)
     double consumption, q_lo, q_hi;
@@ -1708,6 +1708,234 @@ __device__ int place_patches(const WRep& W, abm_patch_state ps, int b, int kind,
     return ABM_WORLD_DONE;
 }
 
+// ------------------------------------------------ agent-agent collisions
+constexpr int kCollideMaxAgents = 2048;   // replicas with collisions on
+constexpr int kCollideMaxNear = 256;      // agents within 2 r + 20 of a colliding agent
+
+size_t collide_lds_bytes(int maxN, int maxR) {
+    return ((sizeof(int32_t) * 5 * (size_t)maxN + sizeof(int32_t) * kCollideMaxNear + 15) & ~(size_t)15) +
+           fg_lds_bytes(kCollideMaxNear, maxR) + 16;
+}
+
+// numpy's pairwise float64 sum (np.add.reduce of a contiguous array,
+// pairwise_sum in numpy/_core/src/umath/loops_utils.h.src): fewer than 8
+// elements in sequence from 0.0; up to 128 in eight strided partial sums
+// combined as ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)), then the
+// remainder in sequence; longer runs split at n / 2 rounded down to a multiple
+// of 8, each half summed the same way.  val(i) is element i.
+template <class Val>
+__device__ double np_block_sum(const Val& val, int lo, int n) {
+#pragma clang fp contract(off)
+    if (n < 8) {
+        double res = 0.0;
+        for (int i = 0; i < n; ++i) res += val(lo + i);
+        return res;
+    }
+    double r[8];
+    for (int j = 0; j < 8; ++j) r[j] = val(lo + j);
+    int i = 8;
+    for (; i < n - (n % 8); i += 8)
+        for (int j = 0; j < 8; ++j) r[j] += val(lo + i + j);
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += val(lo + i);
+    return res;
+}
+
+template <class Val>
+__device__ double np_pairwise_sum(const Val& val, int lo, int n) {
+#pragma clang fp contract(off)
+    // post-order walk of the split tree on an explicit stack
+    int s_lo[32], s_n[32], s_state[32];
+    double s_left[32];
+    int sp = 0;
+    s_lo[0] = lo; s_n[0] = n; s_state[0] = 0;
+    for (;;) {
+        if (s_n[sp] > 128) {
+            const int n2 = s_n[sp] / 2 - (s_n[sp] / 2) % 8;
+            s_state[sp] = 1;
+            s_lo[sp + 1] = s_lo[sp]; s_n[sp + 1] = n2; s_state[sp + 1] = 0;
+            ++sp;
+            continue;
+        }
+        double ret = np_block_sum(val, s_lo[sp], s_n[sp]);
+        for (;;) {
+            if (sp == 0) return ret;   // the node just finished is the root
+            --sp;
+            if (s_state[sp] == 1) {    // its left half: start the right half
+                const int n2 = s_n[sp] / 2 - (s_n[sp] / 2) % 8;
+                s_left[sp] = ret;
+                s_state[sp] = 2;
+                s_lo[sp + 1] = s_lo[sp] + n2; s_n[sp + 1] = s_n[sp] - n2; s_state[sp + 1] = 0;
+                ++sp;
+                break;
+            }
+            ret = s_left[sp] + ret;    // its right half: this node is done
+        }
+    }
+}
+
+// Agent-agent collisions of the loop body (sims.py:736-787), one block per
+// replica before the resource loop.  Collisions are those of
+// pygame.sprite.groupcollide(agents, agents, ..., within_group_collision)
+// with every radius grown by 2 (sims.py:738-747): collide_circle on the rect
+// centres, int(position) + radius (the rect holds int() of the position and
+// is 2 r wide), squared integer distance <= (r1 + 2 + r2 + 2)^2.  Each agent2
+// of a colliding pair (agent1, agent2) takes the proximity response of
+// agent_agent_collision_proximity (sims.py:421-467) once per partner agent1
+// that passes the ghost-mode test; the response reads only agent2's own
+// heading and everyone's positions, which it does not change, so the agents
+// are handled one at a time with their responses in sequence.  Then the
+// collided list (sims.py:760-777) ends or notifies the collide modes
+// (sims.py:779-784).  A replica with an agent beyond kCollideMaxNear
+// neighbours within 2 r + 20 is reported ABM_WORLD_CROWDED and not touched.
+__global__ void __launch_bounds__(256)
+forage_collide_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wreps,
+                      abm_forage_state st, int32_t* __restrict__ pool_success,
+                      const int32_t* __restrict__ pend, int32_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) double smem[];
+    const int b = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    const FGRep& P = reps[b];
+    const WRep& W = wreps[b];
+    if (!W.collide || pend[2 * b] != kPendNone) return;   // uniform over the block
+    const int N = P.n_agents, a0 = P.agent_offset, R = P.n_rays;
+    char* base = reinterpret_cast<char*>(smem);
+    int32_t* cx = reinterpret_cast<int32_t*>(base);
+    int32_t* cy = cx + N;
+    int32_t* rr = cy + N;
+    int32_t* cnt = rr + N;     // responses of each agent as agent2
+    int32_t* flag = cnt + N;   // bit 0 collided, bit 1 exploiting
+    int32_t* near = flag + N;
+    FGSmem S = fg_carve(base + ((sizeof(int32_t) * 5 * (size_t)N + sizeof(int32_t) * kCollideMaxNear + 15) &
+                                ~(size_t)15),
+                        kCollideMaxNear, R);
+    __shared__ int s_crowded, s_nn;
+    if (tid == 0) s_crowded = 0;
+    for (int j = tid; j < N; j += nthr) {
+        const int g = a0 + j;
+        const int r = st.radius[g];
+        cx[j] = (int)trunc(st.x[g]) + r;
+        cy[j] = (int)trunc(st.y[g]) + r;
+        rr[j] = r;
+        flag[j] = get_mode(st.overriding_mode[g], st.w[g], P.T_w) == ABM_MODE_EXPLOIT ? 2 : 0;
+    }
+    __syncthreads();
+    // ---- pairs, responses per agent and the collided list
+    for (int j = tid; j < N; j += nthr) {
+        const bool ex_j = flag[j] & 2;
+        int n_resp = 0;
+        bool any = false, both = false;
+        for (int i = 0; i < N; ++i) {
+            if (i == j) continue;
+            const long dx = cx[i] - cx[j], dy = cy[i] - cy[j], rs = rr[i] + rr[j] + 4;
+            if (dx * dx + dy * dy > rs * rs) continue;
+            const bool ne = !(flag[i] & 2) && !ex_j;
+            any = true;
+            both = both || ne;
+            if (!W.ghost || ne) ++n_resp;
+        }
+        // sims.py:760-777: collided agents (the exploiting status never changes here)
+        const bool collided = W.teleport ? (any && !ex_j) : (!W.ghost ? any : both);
+        cnt[j] = n_resp;
+        flag[j] |= collided ? 1 : 0;
+        if (n_resp > 0) {
+            // neighbours within 2 r + 20 (supcalc.distance, each agent's own radius)
+            const int g = a0 + j;
+            const double rj = (double)rr[j];
+            const double cxj = st.x[g] + rj, cyj = st.y[g] + rj;
+            int nn = 0;
+            for (int i = 0; i < N; ++i) {
+                if (i == j) continue;
+                const int gi = a0 + i;
+                const double ri = (double)rr[i];
+                if (norm2((st.x[gi] + ri) - cxj, (st.y[gi] + ri) - cyj) < 2.0 * rj + 20.0) ++nn;
+            }
+            if (nn > kCollideMaxNear) s_crowded = 1;   // benign race: every writer stores 1
+        }
+    }
+    __syncthreads();
+    if (s_crowded) {
+        if (tid == 0) status[b] = ABM_WORLD_CROWDED;
+        return;   // uniform over the block
+    }
+    // ---- the proximity responses, agent by agent
+    const double fov0 = -kPi, fov1 = kPi;
+    for (int j = 0; j < N; ++j) {
+        if (cnt[j] == 0) continue;   // uniform over the block
+        const int g = a0 + j;
+        const bool ex_j = flag[j] & 2;
+        const double rj = (double)rr[j];
+        const double pxj = st.x[g], pyj = st.y[g];
+        if (tid == 0) {
+            if (!ex_j) { st.overriding_mode[g] = ABM_MODE_COLLIDE; st.mode[g] = ABM_MODE_COLLIDE; }
+            // the agents within 2 r + 20, in id order
+            int nn = 0;
+            for (int i = 0; i < N; ++i) {
+                if (i == j) continue;
+                const int gi = a0 + i;
+                const double ri = (double)rr[i];
+                if (norm2((st.x[gi] + ri) - (pxj + rj), (st.y[gi] + ri) - (pyj + rj)) < 2.0 * rj + 20.0) {
+                    if (nn < kCollideMaxNear) near[nn] = gi;   // the count was checked above
+                    ++nn;
+                }
+            }
+            s_nn = nn < kCollideMaxNear ? nn : kCollideMaxNear;
+        }
+        __syncthreads();
+        const int nn = s_nn;
+        for (int q = tid; q < nn; q += nthr) {
+            S.x[q] = st.x[near[q]];
+            S.y[q] = st.y[near[q]];
+            S.cls[q] = kSocial;
+        }
+        __syncthreads();
+        for (int c = 0; c < cnt[j]; ++c) {
+            const Focal f = make_focal(pxj, pyj, st.theta[g], rj);
+            if (tid == 0) {
+                // the distance left in projection_field's loop variable: its
+                // last non-coincident obstacle (agent.py:501-556)
+                S.lastd[0] = 0.0;
+                for (int q = nn - 1; q >= 0; --q) {
+                    const PairFG p = pair_forage_exact(f, S.x[q], S.y[q], R, P.step, fov0, fov1);
+                    if (!p.skipped) { S.lastd[0] = p.dist; break; }
+                }
+            }
+            project_focal(S, nn, R, P.step, fov0, fov1, P.visual_exclusion != 0, false, f, true);
+            if (tid == 0) {
+                // left / right excitation and the front (sims.py:444-462)
+                const double val2 = 1.0 - S.lastd[0] / P.vision_range;
+                auto val = [&](int r) {
+                    const int code = soc_bit(S.V, r, R, P.step, fov0, fov1);
+                    return code == 0 ? 0.0 : (code == 2 ? val2 : 1.0);
+                };
+                const int h = R / 2;
+                const double left = np_pairwise_sum(val, 0, h) / (double)h;
+                const double right = np_pairwise_sum(val, h, R - h) / (double)(R - h);
+                const double D = left > right ? 1.0 : -1.0;   // np.sign, 0 taken as -1
+                int lo, hi;
+                py_slice(h - 100, h + 100, R, &lo, &hi);
+                const bool front = hi > lo && np_pairwise_sum(val, lo, hi - lo) / (double)(hi - lo) > 0.0;
+                if (!ex_j) st.theta[g] = st.theta[g] - D * 0.2;
+                if (front) st.v[g] = 0.0;
+                else if (!ex_j) st.v[g] = P.exp_vel_max;
+            }
+            __syncthreads();
+        }
+    }
+    // ---- collide modes ended, or notified (sims.py:779-784)
+    for (int j = tid; j < N; j += nthr) {
+        const int g = a0 + j;
+        if (get_mode(st.overriding_mode[g], st.w[g], P.T_w) != ABM_MODE_COLLIDE) continue;
+        if (!(flag[j] & 1)) {
+            st.overriding_mode[g] = -1;
+            st.mode[g] = ABM_MODE_EXPLORE;
+        } else {
+            notify(st, pool_success, P.tau, g, -1, -1);
+        }
+    }
+    if (tid == 0) status[b] = kWorldReady;
+}
+
 // One replica per block.  STEP: agent-resource interaction (sims.py:789-855)
 // then the placement of regenerated patches; FILL: create_resources; RESUME:
 // the placements a previous phase left pending.  Membership is computed for
@@ -1758,6 +1986,11 @@ forage_world_kernel(const FGRep* __restrict__ reps, const WRep* __restrict__ wre
         return;
     }
     if (phase == ABM_WORLD_STEP) {
+        // a replica the collision kernel found crowded is left as it is
+        if (W.collide && status[b] == ABM_WORLD_CROWDED) {
+            if (tid == 0) w_used[b] = 0;
+            return;
+        }
         const int np_ = n_live;
         // the live patches' geometry, quality and units staged in LDS
         __shared__ double s_pr[kMaxPatches], s_px[kMaxPatches], s_py[kMaxPatches];
@@ -2418,7 +2651,7 @@ int abm_forage_world_create(abm_forage_batch* b, const abm_forage_world_params* 
         return rc;
     std::vector<WRep> host(n);
     int64_t total = 0;
-    size_t lds = 8;
+    size_t lds = 8, collide_lds = 0;
     const int64_t kRange = 0xFFFFFFFEll;   // randint ranges up to 2^32 - 2 values
     for (int i = 0; i < n; ++i) {
         const abm_forage_world_params& q = params[i];
@@ -2431,6 +2664,12 @@ int abm_forage_world_create(abm_forage_batch* b, const abm_forage_world_params* 
             return set_error(ABM_ERR_INVALID, "forage world %d: negative patch radius", i);
         w.n_patches = q.n_patches; w.patch_radius = q.patch_radius; w.pooling_time = q.pooling_time;
         w.regenerate = q.regenerate ? 1 : 0; w.teleport = q.teleport_exploit ? 1 : 0;
+        w.collide = q.collide_agents ? 1 : 0; w.ghost = q.ghost_mode ? 1 : 0;
+        if (w.collide && reps[i].n_agents > kCollideMaxAgents)
+            return set_error(ABM_ERR_UNSUPPORTED, "forage world %d: agent-agent collisions for up to %d "
+                             "agents per replica (got %d)", i, kCollideMaxAgents, reps[i].n_agents);
+        if (w.collide)
+            collide_lds = std::max(collide_lds, collide_lds_bytes(reps[i].n_agents, reps[i].n_rays));
         w.patch_offset = total;
         // the bounds numpy sees: int() of the Python expressions (sims.py:349-356)
         const double r = (double)q.patch_radius, pad = reps[i].pad;
@@ -2457,6 +2696,12 @@ int abm_forage_world_create(abm_forage_batch* b, const abm_forage_world_params* 
     }
     abm_forage_world* wd = new abm_forage_world();
     wd->batch = b; wd->n_replicas = n; wd->patch_total = total; wd->member_lds = lds;
+    wd->collide_lds = collide_lds;
+    if (collide_lds > (size_t)163840) {
+        delete wd;
+        return set_error(ABM_ERR_UNSUPPORTED, "forage world: the collision kernel needs %zu B of LDS",
+                         collide_lds);
+    }
     const size_t slots = (size_t)std::max<int64_t>(total, 1);
     int rc = check_hip(hipMalloc(&wd->wreps_dev, sizeof(WRep) * n), "hipMalloc(world replicas)");
     if (!rc) rc = check_hip(hipMalloc(&wd->kill_rank, sizeof(int32_t) * slots), "hipMalloc(kill ranks)");
@@ -2513,6 +2758,19 @@ int world_run(abm_forage_world* w, int32_t phase, const abm_forage_state* st, in
     abm_forage_batch* b = w->batch;
     const size_t lds = w->member_lds;
     if (lds > 65536) return set_error(ABM_ERR_UNSUPPORTED, "%s: %zu B of membership masks", who, lds);
+    if (phase == ABM_WORLD_STEP && w->collide_lds > 0) {
+        // agent-agent collisions first (sims.py:736-787)
+        if (w->collide_lds > 65536) {
+            if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_collide_kernel,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)w->collide_lds), "hipFuncSetAttribute"))
+                return rc;
+        }
+        hipLaunchKernelGGL(forage_collide_kernel, dim3(b->n_replicas), dim3(256), w->collide_lds,
+                           (hipStream_t)stream, (const FGRep*)b->reps_dev, (const WRep*)w->wreps_dev, *st,
+                           pool_success, w->pend, status);
+        if (int rc = check_hip(hipGetLastError(), "forage_collide_kernel launch")) return rc;
+    }
     // a replica above 1024 agents takes a block of 1024 threads for the
     // per-agent parts; smaller ones share a CU in blocks of 256
     const int threads = b->max_agents > 1024 ? 1024 : 256;

@@ -42,6 +42,7 @@ struct abm_forage_world {
     int32_t n_replicas = 0;
     int64_t patch_total = 0;
     size_t member_lds = 0;           // bytes of the membership masks per block
+    size_t collide_lds = 0;          // LDS of the collision kernel (0: no replica collides)
     void* wreps_dev = nullptr;
     int32_t* kill_rank = nullptr;    // [patch_total], -1 = not killed
     int32_t* pend = nullptr;         // [2 * n_replicas]: kind, count

@@ -524,7 +524,8 @@ class VFSimulation:
 
 class Simulation:
     """Headless foraging loop (sims.py:60-221 parameters, 321-374 resources,
-    526-552 creation, 702-870 step) with agent-agent collisions off."""
+    526-552 creation, 702-870 step), agent-agent collisions included
+    (sims.py:421-467, 736-787)."""
 
     def __init__(self, N, T, v_field_res=800, width=600, height=480, framerate=25, window_pad=30,
                  with_visualization=False, show_vis_field=False, show_vis_field_return=False,
@@ -537,10 +538,7 @@ class Simulation:
                  use_zarr=True, allow_border_patch_overlap=False, agent_behave_param_list=None,
                  collide_agents=True, decision_params: DecisionParams = None,
                  movement_params: MovementParams = None):
-        if collide_agents:
-            raise L.AbmError(L.ABM_ERR_UNSUPPORTED,
-                             "agent-agent collisions (AGENT_AGENT_COLLISION=1) are not supported by "
-                             "the GPU loop; pass collide_agents=False")
+        self.collide_agents = collide_agents
         self.N, self.T, self.t = N, T, 0
         self.WIDTH, self.HEIGHT, self.window_pad = width, height, window_pad
         self.allow_border_patch_overlap = allow_border_patch_overlap
@@ -646,12 +644,75 @@ class Simulation:
         cl_ang = (math.atan2(y2 - y1, x2 - x1) + agent.orientation) % (np.pi * 2)
         agent.orientation += (cl_ang - np.pi) * relative_speed
 
+    def agent_agent_collision_proximity(self, agent1, agent2):
+        """Collision response from the proximity field (sims.py:421-467); the
+        field on the GPU (Agent.projection_field with keep_distance_info)."""
+        agents2 = agent2 if isinstance(agent2, list) else [agent2]
+        for agent2 in agents2:
+            do_collision = True
+            if self.ghost_mode:
+                do_collision = agent2.get_mode() != "exploit" and agent1.get_mode() != "exploit"
+            if not do_collision:
+                continue
+            if agent2.get_mode() != "exploit":
+                agent2.set_mode("collide")
+            near = [a for a in self.agents
+                    if self._distance(a, agent2) < 2 * agent2.radius + 20 and a is not agent2]
+            field = agent2.projection_field(near, keep_distance_info=True, fov=(-np.pi, np.pi))
+            n = len(field)
+            d_lr = np.sign(np.mean(field[0:int(n / 2)]) - np.mean(field[int(n / 2)::]))
+            if d_lr == 0:
+                d_lr = -1
+            if agent2.get_mode() != "exploit":
+                agent2.orientation -= d_lr * 0.2
+            if np.mean(field[int(n / 2) - 100:int(n / 2) + 100]) > 0:
+                agent2.velocity = 0
+            elif agent2.get_mode() != "exploit":
+                agent2.velocity = agent2.max_exp_vel
+
+    def collide(self):
+        """Agent-agent collisions (sims.py:736-787): groupcollide of the agents
+        with themselves, collide_circle on the rect centres (int(position) +
+        radius, pygame's Rect holding int() of the position) with every radius
+        grown by 2; the responses in group order; collide modes ended or
+        notified."""
+        agents = self.agents.sprites()
+        cent = [(int(a.position[0]) + a.radius, int(a.position[1]) + a.radius) for a in agents]
+        groups = []
+        for i, a in enumerate(agents):
+            hits = [b for j, b in enumerate(agents) if j != i and
+                    (cent[i][0] - cent[j][0]) ** 2 + (cent[i][1] - cent[j][1]) ** 2
+                    <= (a.radius + 2 + b.radius + 2) ** 2]
+            if hits:
+                groups.append((a, hits))
+        collided = []
+        for agent1, agents2 in groups:
+            self.agent_agent_collision_proximity(agent1, agents2)
+            for agent2 in agents2:
+                if self.teleport_exploit:
+                    if agent1.get_mode() != "exploit":
+                        collided.append(agent1)
+                    if agent2.get_mode() != "exploit":
+                        collided.append(agent2)
+                elif not self.ghost_mode:
+                    collided += [agent1, agent2]
+                elif agent1.get_mode() != "exploit" and agent2.get_mode() != "exploit":
+                    collided += [agent1, agent2]
+        for a in agents:
+            hit = any(a is c for c in collided)
+            if not hit and a.get_mode() == "collide":
+                a.set_mode("explore")
+            if hit and a.get_mode() == "collide":
+                _notify_agent(a, -1)
+
     def step(self):
-        """One step of the loop body (sims.py:789-870)."""
+        """One step of the loop body (sims.py:736-870)."""
         if self.is_paused:
             for ag in self.agents:
                 ag.calc_social_V_proj(self.agents)
             return
+        if self.collide_agents:
+            self.collide()
         agents = self.agents.sprites()
         groups = [(r, [a for a in agents if self._distance(r, a) < r.radius]) for r in self.rescources]
         on_patch = []

@@ -272,6 +272,8 @@ class WorldConfig:
     agent_consumption: float = 1.0
     teleport_exploit: bool = True
     allow_border_patch_overlap: bool = False
+    collide_agents: bool = False
+    ghost_mode: bool = True
 
     def to_c(self) -> L.ForageWorldParams:
         max_q = self.min_resc_quality if self.max_resc_quality < 0 else self.max_resc_quality
@@ -282,6 +284,7 @@ class WorldConfig:
             pooling_time=int(self.pooling_time), regenerate=int(bool(self.regenerate_patches)),
             teleport_exploit=int(bool(self.teleport_exploit)),
             allow_border_overlap=int(bool(self.allow_border_patch_overlap)),
+            collide_agents=int(bool(self.collide_agents)), ghost_mode=int(bool(self.ghost_mode)),
             consumption=float(self.agent_consumption), min_quality=float(self.min_resc_quality),
             max_quality=float(max_q))
 
@@ -290,6 +293,8 @@ _PATCH_F64 = ("x", "y", "radius", "units", "left", "quality")
 _MT_WORDS = 625                       # abm_mt19937_state: 624 key words and the position
 _MT_BYTES = 4 * _MT_WORDS
 TIMEOUT_MESSAGE = "Reached timeout while trying to create resources without overlap!"
+CROWDED_MESSAGE = ("a colliding agent has more than 256 agents within 2 r + 20 (the collision "
+                   "kernel's proximity limit); the replica was not stepped")
 
 
 class ResourceWorld:
@@ -374,6 +379,8 @@ class ResourceWorld:
             raise Exception(TIMEOUT_MESSAGE)
         if np.any(status == L.WORLD_BAD_COUNT):
             raise L.AbmError(L.ABM_ERR_INVALID, "a patch count is outside [0, N_resc]")
+        if np.any(status == L.WORLD_CROWDED):
+            raise L.AbmUnsupported(L.ABM_ERR_UNSUPPORTED, CROWDED_MESSAGE)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -454,6 +461,8 @@ class ResourceWorld:
                         raise Exception(TIMEOUT_MESSAGE)
                     if np.any(status == L.WORLD_BAD_COUNT):
                         raise L.AbmError(L.ABM_ERR_INVALID, "a patch count is outside [0, N_resc]")
+                    if np.any(status == L.WORLD_CROWDED):
+                        raise L.AbmUnsupported(L.ABM_ERR_UNSUPPORTED, CROWDED_MESSAGE)
                     need = status == L.WORLD_NEED_WORDS
                     if not need.any():
                         return

@@ -428,8 +428,14 @@ typedef struct abm_mt19937_state {
 
 /* ------------------------------------------- agent-resource interaction */
 /* The resource patches of every foraging replica and the loop body around
- * Agent.update that uses them (abm/simulation/sims.py:789-861 with agent-agent
- * collisions off): patch membership of agent centres (groupcollide +
+ * Agent.update that uses them (abm/simulation/sims.py:736-861): agent-agent
+ * collisions when on (sims.py:736-787: pygame.sprite.groupcollide with
+ * collide_circle on the agents' rect centres, int(position) + radius, with
+ * every radius grown by 2; the proximity response of
+ * agent_agent_collision_proximity, sims.py:421-467, from the
+ * keep_distance_info projection field of the agents within 2 r + 20; the
+ * collide modes ended or notified, replicas of up to 2048 agents), patch
+ * membership of agent centres (groupcollide +
  * refine_ar_overlap_group, sims.py:45-56, 790-799), bias towards the patch
  * centre (bias_agent_towards_res_center, sims.py:544-552), notify_agent
  * (sims.py:29-42), teleport, Rescource.deplete (abm/environment/rescource.py:
@@ -445,6 +451,8 @@ typedef struct abm_forage_world_params {
     int32_t regenerate;           /* REGENERATE_PATCHES                               */
     int32_t teleport_exploit;     /* TELEPORT_TO_MIDDLE                               */
     int32_t allow_border_overlap; /* ALLOW_BORDER_PATCH_OVERLAP                       */
+    int32_t collide_agents;       /* AGENT_AGENT_COLLISION                            */
+    int32_t ghost_mode;           /* GHOST_WHILE_EXPLOIT                              */
     double  consumption;          /* AGENT_CONSUMPTION                                */
     double  min_quality, max_quality;  /* np.random.uniform bounds (max fixed up)     */
 } abm_forage_world_params;
@@ -482,6 +490,9 @@ int abm_forage_world_get_offsets(const abm_forage_world* world, int64_t* patch_o
                                     raises "Reached timeout while trying to create
                                     resources without overlap!")                   */
 #define ABM_WORLD_BAD_COUNT  4   /* count[b] outside [0, n_patches]: nothing done    */
+#define ABM_WORLD_CROWDED    5   /* a colliding agent has more than 256 agents within
+                                    2 r + 20 (the proximity field's limit): nothing
+                                    done                                            */
 
 /* One phase for every replica.  words: per replica the next raw 32-bit outputs
  * of its numpy legacy MT19937 stream (abm_mt19937_peek32), word_offset[b] /

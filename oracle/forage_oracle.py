@@ -460,12 +460,13 @@ def _collide_circle(c1, r1, c2, r2):
 
 
 class ForageWorld:
-    """Simulation state for the headless foraging loop (sims.py:702-870) with
-    agent collisions off (AGENT_AGENT_COLLISION=0)."""
+    """Simulation state for the headless foraging loop (sims.py:702-870);
+    agent-agent collisions (sims.py:736-787) when ``collide_agents``."""
 
     def __init__(self, agents, W, H, pad, n_resc, resc_radius, min_units, max_units,
                  min_quality, max_quality, regenerate=True, teleport_exploit=False,
-                 allow_border_overlap=False, agent_consumption=1):
+                 allow_border_overlap=False, agent_consumption=1, collide_agents=False,
+                 ghost_mode=True):
         self.agents = agents
         self.W, self.H, self.pad = W, H, pad
         self.resc_radius = resc_radius
@@ -479,6 +480,8 @@ class ForageWorld:
         self.teleport_exploit = teleport_exploit
         self.allow_border_overlap = allow_border_overlap
         self.consumption = agent_consumption
+        self.collide_agents = collide_agents
+        self.ghost_mode = ghost_mode
         self.patches: List[Patch] = []
         for _ in range(n_resc):
             self.add_patch()
@@ -512,9 +515,78 @@ class ForageWorld:
         self.patches.append(new)
         return new.id
 
+    def rect_centre(self, a):
+        """The agent's pygame rect centre: rect.x, rect.y = position
+        (agent.py:303-304, pygame's Rect stores int(x), a C cast: truncation)
+        and a 2r x 2r image (agent.py:153), so the centre is int(position) + r."""
+        return int(a.position[0]) + a.radius, int(a.position[1]) + a.radius
+
+    def proximity(self, agent1, agents2):
+        """Simulation.agent_agent_collision_proximity (sims.py:421-467)."""
+        for agent2 in agents2:
+            do_collision = True
+            if self.ghost_mode:
+                do_collision = agent2.get_mode() != "exploit" and agent1.get_mode() != "exploit"
+            if not do_collision:
+                continue
+            if agent2.get_mode() != "exploit":
+                agent2.set_mode("collide")
+            near = [a for a in self.agents if distance(a, agent2) < 2 * agent2.radius + 20 and a is not agent2]
+            field = projection_field(agent2, near, keep_distance_info=True, fov=(-np.pi, np.pi))
+            L = len(field)
+            left = np.mean(field[0:int(L / 2)])
+            right = np.mean(field[int(L / 2)::])
+            D = np.sign(left - right)
+            if D == 0:
+                D = -1
+            if agent2.get_mode() != "exploit":
+                agent2.orientation -= D * 0.2
+            if np.mean(field[int(L / 2) - 100:int(L / 2) + 100]) > 0:
+                agent2.velocity = 0
+            elif agent2.get_mode() != "exploit":
+                agent2.velocity = agent2.max_exp_vel
+
+    def collide(self):
+        """Agent-agent collisions (sims.py:736-787): pygame.sprite.groupcollide
+        of the agents with themselves (interactions.py:5-10, collide_circle on
+        the rect centres with every radius grown by 2, sims.py:738-747), the
+        proximity response per colliding pair in group order, then the
+        collide modes ended or notified."""
+        cent = [self.rect_centre(a) for a in self.agents]
+        groups = []
+        for i, a in enumerate(self.agents):
+            hits = [b for j, b in enumerate(self.agents)
+                    if j != i and _collide_circle(cent[i], a.radius + 2, cent[j], b.radius + 2)]
+            if hits:
+                groups.append((a, hits))
+        collided = []
+        for agent1, agents2 in groups:
+            self.proximity(agent1, agents2)
+            for agent2 in agents2:
+                if self.teleport_exploit:
+                    if agent1.get_mode() != "exploit":
+                        collided.append(agent1)
+                    if agent2.get_mode() != "exploit":
+                        collided.append(agent2)
+                elif not self.ghost_mode:
+                    collided.append(agent1)
+                    collided.append(agent2)
+                elif agent1.get_mode() != "exploit" and agent2.get_mode() != "exploit":
+                    collided.append(agent1)
+                    collided.append(agent2)
+        for a in self.agents:
+            hit = any(a is c for c in collided)
+            if not hit and a.get_mode() == "collide":
+                a.set_mode("explore")
+            if hit and a.get_mode() == "collide":
+                notify_agent(a, -1)
+
     def step(self):
-        """One simulation step: agent-resource interaction (sims.py:789-858),
-        then the agents' update (sims.py:861)."""
+        """One simulation step: agent-agent collisions when on (sims.py:736-787),
+        agent-resource interaction (sims.py:789-858), then the agents' update
+        (sims.py:861)."""
+        if self.collide_agents:
+            self.collide()
         on_patch = []
         # groupcollide + refine_ar_overlap_group (sims.py:45-56, 790-799): the
         # membership of every patch is fixed before the loop

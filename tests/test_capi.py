@@ -142,7 +142,8 @@ def test_mt19937_peek32_and_skip32_give_numpys_raw_words():
 
 
 def test_world_struct_layouts():
-    assert C.sizeof(L.ForageWorldParams) == 8 * 4 + 3 * 8
+    assert C.sizeof(L.ForageWorldParams) == 10 * 4 + 3 * 8
+    assert L.ForageWorldParams.consumption.offset == 40
     assert C.sizeof(L.PatchState) == 8 * 8
 
 

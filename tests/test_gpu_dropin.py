@@ -55,10 +55,12 @@ def test_vf_simulation_lockstep(gpu):
         assert np.array_equal(a.soc_v_field, vo.calc_soc_v_proj(o, orc))
 
 
-@pytest.mark.parametrize("vis_ex", [False, True])
-def test_forage_simulation_lockstep(gpu, vis_ex):
+@pytest.mark.parametrize("vis_ex,collide", [(False, False), (True, False), (False, True)])
+def test_forage_simulation_lockstep(gpu, vis_ex, collide):
     # C1 shape: N=10, r=10, R=1200, 10 patches of radius 30, quality U[0.5, 1],
-    # 500x500 walls, vision range 2000; .env decision/movement values
+    # 500x500 walls, vision range 2000; .env decision/movement values.  With
+    # agent-agent collisions on (sims.py:736-787) the arena is 150x150 with 3
+    # patches, so that agents collide every few steps.
     from abm_amd.dropin import DecisionParams, MovementParams, Simulation
     env = {"DEC_TW": 0.5, "DEC_EPSW": 3, "DEC_GW": 0.085, "DEC_BW": 0, "DEC_WMAX": 1, "DEC_TU": 0.5,
            "DEC_EPSU": 1, "DEC_GU": 0.085, "DEC_BU": 0, "DEC_UMAX": 1, "DEC_SWU": 0.25,
@@ -67,12 +69,13 @@ def test_forage_simulation_lockstep(gpu, vis_ex):
            "CONS_STOP_RATIO": 0.175}
     env = {k: str(v) for k, v in env.items()}
     dp, mp = DecisionParams.from_env(env), MovementParams.from_env(env)
-    np.random.seed(3 + vis_ex)
-    sim = Simulation(N=10, T=0, v_field_res=1200, width=500, height=500, pooling_time=0,
-                     agent_radius=10, N_resc=10, min_resc_perpatch=100, max_resc_perpatch=-1,
+    np.random.seed(3 + vis_ex + 2 * collide)
+    side, n_resc = (150, 3) if collide else (500, 10)
+    sim = Simulation(N=10, T=0, v_field_res=1200, width=side, height=side, pooling_time=0,
+                     agent_radius=10, N_resc=n_resc, min_resc_perpatch=100, max_resc_perpatch=-1,
                      min_resc_quality=0.5, max_resc_quality=1.0, patch_radius=30,
                      teleport_exploit=False, vision_range=2000, agent_fov=1.0,
-                     visual_exclusion=vis_ex, collide_agents=False, decision_params=dp,
+                     visual_exclusion=vis_ex, collide_agents=collide, decision_params=dp,
                      movement_params=mp)
     sim.create_agents()
     sim.create_resources()
@@ -84,9 +87,10 @@ def test_forage_simulation_lockstep(gpu, vis_ex):
                          exp_theta_max=mp.exp_theta_max, reloc_theta_max=mp.reloc_theta_max,
                          exp_stop_ratio=mp.exp_stop_ratio)
     orc = [fo.ForageAgent(a.id, a.radius, a.position.copy(), a.orientation, 1200, sim.agent_fov,
-                          500, 500, 30, 2000, vis_ex, True, op) for a in ags]
+                          side, side, 30, 2000, vis_ex, True, op) for a in ags]
     world = fo.ForageWorld.__new__(fo.ForageWorld)
-    world.agents, world.W, world.H, world.pad = orc, 500, 500, 30
+    world.agents, world.W, world.H, world.pad = orc, side, side, 30
+    world.collide_agents, world.ghost_mode = collide, True
     world.resc_radius, world.min_units, world.max_units = 30, 100, 101
     world.min_quality, world.max_quality = 0.5, 1.0
     world.regenerate, world.teleport_exploit, world.allow_border_overlap = True, False, False
@@ -113,3 +117,5 @@ def test_forage_simulation_lockstep(gpu, vis_ex):
         assert [(r.id, r.resc_left) for r in sim.rescources] == [(p.id, p.resc_left) for p in world.patches]
         collected = sum(a.collected_r for a in ags)
     assert {"explore", "exploit"} <= modes_seen and collected > 0
+    if collide:
+        assert "collide" in modes_seen

@@ -65,10 +65,10 @@ def make_agents(seed, N, W, R, mixed, pooling_time):
     return ags, p
 
 
-def forage_config(N, W, R, p):
+def forage_config(N, W, R, p, vis_ex=False):
     from abm_amd.forage import ForageConfig
     return ForageConfig(n_agents=N, n_rays=R, width=W, height=W, window_pad=30, fov=(-PI, PI),
-                        vision_range=150.0, visual_exclusion=False, patchwise_exclusion=True,
+                        vision_range=150.0, visual_exclusion=vis_ex, patchwise_exclusion=True,
                         T_w=p.T_w, Eps_w=p.Eps_w, g_w=p.g_w, B_w=p.B_w, w_max=p.w_max, T_u=p.T_u,
                         Eps_u=p.Eps_u, g_u=p.g_u, B_u=p.B_u, u_max=p.u_max, S_wu=p.S_wu,
                         S_uw=p.S_uw, Tau=p.Tau, F_N=p.F_N, F_R=p.F_R, exp_vel_max=p.exp_vel_max,
@@ -83,7 +83,8 @@ def oracle_world(ags, W, wk):
                           c.max_resc_perpatch, c.min_resc_quality, c.max_resc_quality,
                           regenerate=c.regenerate_patches, teleport_exploit=c.teleport_exploit,
                           allow_border_overlap=c.allow_border_patch_overlap,
-                          agent_consumption=c.agent_consumption)
+                          agent_consumption=c.agent_consumption, collide_agents=c.collide_agents,
+                          ghost_mode=c.ghost_mode)
 
 
 def load_agents(sw, rw, ags, b):
@@ -444,3 +445,91 @@ def test_world_overlapping_caller_patches(gpu):
             compare(sw, rw, world, b, (step, b))
             load_agents(sw, rw, world.agents, b)
     assert shared > 0
+
+
+# (N agents, arena, R, WorldConfig kwargs, visual exclusion): crowded arenas
+# with agent-agent collisions on
+COLLIDE_WORLDS = [
+    (40, 220, 256, dict(N_resc=3, patch_radius=30, min_resc_perpatch=3, max_resc_perpatch=8,
+                        min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=True,
+                        collide_agents=True, ghost_mode=True), False),
+    (30, 180, 1200, dict(N_resc=2, patch_radius=25, min_resc_perpatch=2, max_resc_perpatch=5,
+                         min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False,
+                         collide_agents=True, ghost_mode=False), False),
+    (35, 200, 512, dict(N_resc=3, patch_radius=30, min_resc_perpatch=3, max_resc_perpatch=6,
+                        min_resc_quality=0.5, max_resc_quality=1.0, teleport_exploit=False,
+                        collide_agents=True, ghost_mode=True), True),
+    # collisions off next to them
+    (20, 250, 256, dict(N_resc=3, patch_radius=30, min_resc_perpatch=3, max_resc_perpatch=6,
+                        teleport_exploit=True), False),
+]
+
+
+@pytest.mark.parametrize("device_rng", [False, True])
+def test_world_collisions_match_oracle(gpu, device_rng):
+    # sims.py:736-787 (groupcollide on rect centres, the proximity response,
+    # the collided list) before the resource loop, every step from the
+    # oracle's state: discrete state identical, continuous <= 1e-12
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    worlds, cfgs, wcfgs, allags = [], [], [], []
+    for b, (N, W, R, wk, vis_ex) in enumerate(COLLIDE_WORLDS):
+        ags, p = make_agents(300 + b, N, W, R, True, 0)
+        for a in ags:
+            a.visual_exclusion = vis_ex
+        worlds.append((ags, W, wk))
+        cfgs.append(forage_config(N, W, R, p, vis_ex))
+        wcfgs.append(WorldConfig(**wk))
+        allags += ags
+    rngs = [np.random.RandomState(950 + b) for b in range(len(worlds))]
+    orngs = [np.random.RandomState(950 + b) for b in range(len(worlds))]
+    sw = ForageSwarm(cfgs, [a.position[0] for a in allags], [a.position[1] for a in allags],
+                     [a.orientation for a in allags], radius=10, rngs=rngs)
+    rw = ResourceWorld(sw, wcfgs, device_rng=device_rng)
+    oracle = []
+    for b, (ags, W, wk) in enumerate(worlds):
+        load_agents(sw, rw, ags, b)
+        np.random.set_state(orngs[b].get_state())
+        oracle.append(oracle_world(ags, W, wk))
+        orngs[b].set_state(np.random.get_state())
+    rw.create_resources()
+    rw.sync_rng()
+    collide_seen = [0] * len(worlds)
+    for step in range(40):
+        print(f"  collisions step {step} device_rng={device_rng}", flush=True)
+        rw.step(fields=step % 4 == 0)
+        rw.sync_rng()
+        for b, world in enumerate(oracle):
+            np.random.set_state(orngs[b].get_state())
+            world.step()
+            orngs[b].set_state(np.random.get_state())
+            collide_seen[b] += sum(a.mode == "collide" for a in world.agents)
+            assert same_stream(rngs[b], orngs[b]), (step, b)
+            compare(sw, rw, world, b, (step, b))
+            load_agents(sw, rw, world.agents, b)
+    assert all(c > 0 for c in collide_seen[:3]) and collide_seen[3] == 0, collide_seen
+
+
+def test_world_collisions_crowded_and_size_limits(gpu):
+    # more than 256 agents within 2 r + 20 of a colliding agent: the replica
+    # is reported and not stepped; collisions above 2048 agents are refused
+    from abm_amd import AbmUnsupported
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    N, W, R = 300, 400, 64
+    ags, p = make_agents(5, N, W, R, False, 0)
+    rng = np.random.RandomState(6)
+    for a in ags:
+        a.position = np.array([200.0 + rng.randint(0, 20), 200.0 + rng.randint(0, 20)])
+    rngs = [np.random.RandomState(7)]
+    sw = ForageSwarm([forage_config(N, W, R, p)], [a.position[0] for a in ags],
+                     [a.position[1] for a in ags], [a.orientation for a in ags], radius=10, rngs=rngs)
+    rw = ResourceWorld(sw, [WorldConfig(N_resc=1, collide_agents=True)])
+    load_agents(sw, rw, ags, 0)
+    rw.create_resources()
+    with pytest.raises(AbmUnsupported, match="2 r \\+ 20"):
+        rw.step()
+    big, _ = make_agents(8, 2049, 2000, 64, False, 0)
+    sw2 = ForageSwarm([forage_config(2049, 2000, 64, p)], [a.position[0] for a in big],
+                      [a.position[1] for a in big], [a.orientation for a in big], radius=10,
+                      rngs=[np.random.RandomState(9)])
+    with pytest.raises(AbmUnsupported, match="collisions"):
+        ResourceWorld(sw2, [WorldConfig(N_resc=1, collide_agents=True)])

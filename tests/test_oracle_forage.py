@@ -563,3 +563,105 @@ def test_bias_multiplies_heading_differences_by_1_02_per_step():
         thetas.append(ags[0].orientation)
     assert math.isclose(thetas[1] - thetas[0], 1e-9 * 1.02 ** 50, rel_tol=1e-5)
     assert math.isclose(thetas[0] - PI, 0.1 * 1.02 ** 50, rel_tol=1e-9)
+
+
+# ------------------------------------------------ agent-agent collisions
+def _collide_world(positions, thetas=None, R=1200, **kw):
+    p = fo.ForageParams()
+    thetas = thetas or [0.0] * len(positions)
+    ags = [fo.ForageAgent(i, 10, pos, th, R, (-PI, PI), 500, 500, 30, 150.0, False, True, p)
+           for i, (pos, th) in enumerate(zip(positions, thetas))]
+    for a in ags:
+        a.velocity = 0.5
+    w = fo.ForageWorld(ags, 500, 500, 30, 0, 30, 3, 6, 0.5, 1.0, collide_agents=True, **kw)
+    return w, ags
+
+
+@pytest.mark.parametrize("p1,hit", [((124.0, 100.0), True), ((124.999, 100.0), True),
+                                    ((125.0, 100.0), False), ((116.0, 117.0), True),
+                                    ((117.0, 117.0), False), ((100.0, 76.0), True)])
+def test_collisions_use_truncated_rect_centres_and_radii_grown_by_2(p1, hit):
+    # sims.py:738-747 + pygame collide_circle: rect centres int(position) + r,
+    # radii 10 + 2 each: squared distance <= 24^2 = 576 (17^2 + 17^2 = 578 > 576)
+    w, (a, b) = _collide_world([(100.0, 100.0), p1])
+    w.collide()
+    assert (a.get_mode() == "collide") == hit and (b.get_mode() == "collide") == hit
+    assert (a.env_status == -1) == hit   # collided agents in collide mode are told -1
+
+
+def test_object_dead_ahead_is_invisible_so_the_tie_turns_by_plus_0_2():
+    # the other agent 24 px straight ahead: angle 0 makes the closed angle
+    # 2 pi (agent.py:520-523), outside the strict FOV test, so the field is
+    # empty: left and right excitations tie, np.sign gives 0, taken as -1
+    # (sims.py:449-450): theta += 0.2; the front is clear: velocity =
+    # max_exp_vel
+    w, (a, b) = _collide_world([(100.0, 100.0), (124.0, 100.0)])
+    w.collide()
+    assert a.orientation == 0.2 and a.velocity == a.max_exp_vel
+
+
+def test_object_ahead_off_axis_turns_away_and_stops():
+    # agent 1 at (124, 99): rect centres 24 and 1 apart, 577 > 576, no
+    # collision of its own, but 24.02 px away (inside 2 r + 20); agent 2 right
+    # behind (invisible, closed angle pi) collides with agent 0.  Agent 1's
+    # closed angle 0.0416 -> bin 607, proj_size 150.7 -> rays [531, 682),
+    # flipped 518..668: 82 rays left of 600, 69 right: D = +1, theta -= 0.2;
+    # the front window [500, 700) is covered: velocity 0
+    w, ags = _collide_world([(100.0, 100.0), (124.0, 99.0), (76.0, 100.0)])
+    w.collide()
+    assert ags[0].orientation == -0.2 and ags[0].velocity == 0
+    assert ags[1].get_mode() != "collide"
+
+
+@pytest.mark.parametrize("p1,dth,stop", [((117.0, 83.0), -0.2, True),   # ahead, screen up: left half
+                                         ((117.0, 117.0), 0.2, True),   # mirror image: right half
+                                         ((83.0, 83.0), -0.2, False)])  # behind, up: left half, front clear
+def test_proximity_turn_sign_and_front_window(p1, dth, stop):
+    # the other agent 17 px off in x and y (centres 24.04 px apart, inside
+    # 2 r + 20 = 40, rect centres 17, 17 -> 578 > 576: no collision of its
+    # own), so agent 0 collides with agent 2 straight behind it at 24 px and
+    # takes one response with agents 1 and 2 in its field
+    w, ags = _collide_world([(100.0, 100.0), p1, (76.0, 100.0)])
+    w.collide()
+    a = ags[0]
+    # agent 2 behind (closed angle pi) is outside the strict FOV test, so the
+    # field holds agent 1 only: +pi/4 (up) -> flipped rays 376..526 (left
+    # half, front window reached), 3 pi/4 (behind, up) -> rays 76..226
+    assert a.orientation == pytest.approx(dth, abs=1e-15)
+    assert (a.velocity == 0) == stop
+    if not stop:
+        assert a.velocity == a.max_exp_vel
+
+
+def test_two_partners_give_two_responses():
+    # agent 0 collides with agent 1 straight ahead and agent 2 straight
+    # behind, both invisible: two responses (one per groupcollide partner),
+    # each a tie turning by +0.2
+    w, ags = _collide_world([(100.0, 100.0), (124.0, 100.0), (76.0, 100.0)])
+    w.collide()
+    assert ags[0].orientation == pytest.approx(0.4, abs=1e-15)
+    assert ags[1].orientation == 0.2 and ags[2].orientation == 0.2
+
+
+@pytest.mark.parametrize("ghost,teleport", [(True, False), (False, False), (True, True)])
+def test_ghost_mode_and_collided_list(ghost, teleport):
+    # agent 1 exploits (sims.py:431-436, 760-784).  Ghost mode: no response
+    # for the pair; agent 0 is collided only with teleport (that branch adds
+    # every non-exploiting agent), so its old collide mode ends (explore)
+    # without it and is told -1 with it.  Without ghost mode: agent 0 takes
+    # the (tie) response and is told -1; the exploiter keeps its heading and
+    # mode, and with the front clear its speed too.
+    w, (a, b) = _collide_world([(100.0, 100.0), (124.0, 100.0)], ghost_mode=ghost, teleport_exploit=teleport)
+    a.set_mode("collide")
+    b.set_mode("exploit")
+    w.collide()
+    if ghost:
+        assert a.orientation == 0.0
+        if teleport:
+            assert a.get_mode() == "collide" and a.env_status == -1
+        else:
+            assert a.get_mode() == "explore" and a.env_status == 0
+    else:
+        assert a.get_mode() == "collide" and a.orientation == 0.2 and a.env_status == -1
+        assert b.get_mode() == "exploit" and b.orientation == 0.0 and b.velocity == 0.5
+    assert b.env_status == 0
