@@ -533,3 +533,54 @@ def test_world_collisions_crowded_and_size_limits(gpu):
                       rngs=[np.random.RandomState(9)])
     with pytest.raises(AbmUnsupported, match="collisions"):
         ResourceWorld(sw2, [WorldConfig(N_resc=1, collide_agents=True)])
+
+
+def test_collision_response_on_mirror_images(gpu):
+    # the proximity response's left/right decision where the two halves hold
+    # mirror images (two neighbours at +-dy, 2 r + 20 apart at most, no
+    # collision of their own), so the means differ only by the order of
+    # numpy's pairwise sums; a colliding partner straight behind (invisible)
+    # triggers the response.  R = 100 puts the front window's start before 0
+    # (a Python slice from the end).
+    from abm_amd.forage import ForageSwarm, ResourceWorld, WorldConfig
+    cases = [(dx, dy, R) for R in (1200, 1024, 256, 100) for dx, dy in ((10, 18), (21, 12), (5, 25), (16, 17))]
+    worlds, allags, cfgs = [], [], []
+    p = None
+    for b, (dx, dy, R) in enumerate(cases):
+        ags, p = make_agents(400 + b, 4, 300, R, False, 0)
+        pos = [(100.0, 100.0), (76.0, 100.0), (100.0 + dx, 100.0 - dy), (100.0 + dx, 100.0 + dy)]
+        for a, q in zip(ags, pos):
+            a.position = np.array(q)
+            a.orientation = 0.0
+            a.velocity = 0.5
+        worlds.append(ags)
+        cfgs.append(forage_config(4, 300, R, p))
+        allags += ags
+    rngs = [np.random.RandomState(1000 + b) for b in range(len(cases))]
+    orngs = [np.random.RandomState(1000 + b) for b in range(len(cases))]
+    sw = ForageSwarm(cfgs, [a.position[0] for a in allags], [a.position[1] for a in allags],
+                     [a.orientation for a in allags], radius=10, rngs=rngs)
+    wk = dict(N_resc=1, patch_radius=20, min_resc_perpatch=3, max_resc_perpatch=6,
+              collide_agents=True, ghost_mode=True, teleport_exploit=False)
+    rw = ResourceWorld(sw, [WorldConfig(**wk)] * len(cases))
+    oracle = []
+    for b, ags in enumerate(worlds):
+        load_agents(sw, rw, ags, b)
+        np.random.set_state(orngs[b].get_state())
+        oracle.append(oracle_world(ags, 300, wk))
+        orngs[b].set_state(np.random.get_state())
+    rw.create_resources()   # the same draws as the oracle's patch
+    for b in range(len(cases)):
+        # the patch away from the agents
+        oracle[b].patches[0].position = np.array([250.0, 250.0])
+        oracle[b].patches[0].center = (270.0, 270.0)
+        q = oracle[b].patches[0]
+        rw.set_patches(b, [q.id], [250.0], [250.0], [q.radius], [q.resc_units], [q.resc_left],
+                       [q.unit_per_timestep])
+    rw.step()
+    for b, world in enumerate(oracle):
+        np.random.set_state(orngs[b].get_state())
+        world.step()
+        orngs[b].set_state(np.random.get_state())
+        assert world.agents[0].mode == "collide"
+        compare(sw, rw, world, b, cases[b])
