@@ -296,9 +296,10 @@ static __device__ __noinline__ int resolve_queued(int e, int fc, float fx, float
 // picks b + 1 for s > 0 and b for s < 0.  Returns the bin, or -1 when the pair
 // lies within 1e-7 rad of the tie and the float64 replica must decide.
 // sin and cos in float64 for |x| < 1e5: Cody-Waite reduction by pi/2 with a
-// two-part constant (the first part has 33 significant bits, so k * part is
-// exact for |k| < 2^20 and x - k * part is exact), then the Taylor series of
-// sin to r^11 and of cos to r^12 on |r| <= pi/4 (truncation below 1e-11).
+// two-part constant (fdlibm's published pio2_1 / pio2_1t split: the first
+// part has 33 significant bits, so k * part is exact for |k| < 2^20 and
+// x - k * part is exact), then the Taylor series (coefficients 1/n!) of sin
+// to r^11 and of cos to r^12 on |r| <= pi/4 (truncation below 1e-11).
 // Returns false (outputs unset) for larger arguments or NaN.
 __device__ __forceinline__ bool sincos_f64_small(double x, double* s, double* c) {
     if (!(fabs(x) < 1e5)) return false;
