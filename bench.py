@@ -785,7 +785,11 @@ def main():
             "replicas": n_rep, "agents_per_replica": N, "rays": R,
         },
         "plain_f64": {"agent_steps_per_s": rate64, "ms_per_step": ms64_mean,
-                      "steps": args.steps_f64},
+                      "steps": args.steps_f64,
+                      "roofline_frac": ops / (ms64_mean * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                      "note": ("the same lane-op count over the float64 kernel's time; its float64 "
+                               "transcendentals and one wave per replica keep VALUBusy near 58 % "
+                               "(rocprofv3), against about 82 % for the fused kernel")},
         "speedup_vs_plain_f64": rate32 / rate64,
         "fixed_error": {
             "one_step_rel_err_vs_f64": float(e_sync.max()),
