@@ -299,11 +299,12 @@ CROWDED_MESSAGE = ("a colliding agent has more than 256 agents within 2 r + 20 (
 
 class ResourceWorld:
     """Resource patches of every replica of a ``ForageSwarm`` and the whole
-    loop body of ``Simulation.start`` around them (sims.py:789-861, agent
-    collisions off) on the device: ``create_resources`` places the initial
-    patches (sims.py:538-542), ``step`` runs the agent-resource interaction,
-    the regeneration of depleted patches and ``Agent.update`` for all
-    replicas.
+    loop body of ``Simulation.start`` around them (sims.py:736-861) on the
+    device: ``create_resources`` places the initial patches (sims.py:538-542),
+    ``step`` runs the agent-agent collisions of replicas whose
+    ``WorldConfig.collide_agents`` is set (sims.py:736-787), the agent-resource
+    interaction, the regeneration of depleted patches and ``Agent.update`` for
+    all replicas.
 
     Random numbers: each replica's numpy generator is peeked as raw 32-bit
     words (patch placement draws randint and uniform from them, the random
@@ -316,7 +317,12 @@ class ResourceWorld:
     no host work or synchronisation.  The numpy generators are then stale
     until ``sync_rng()`` writes the device states back into them (call it
     before anything else draws from them); ``check()`` reads the replicas'
-    status and raises the reference's exception for a placement timeout."""
+    status and raises the reference's exception for a placement timeout, or
+    the crowded-collision error.  Steps in this mode do not read the status:
+    a replica whose placements timed out (or whose collisions were too
+    crowded) is skipped, and stays frozen on every later step, until
+    ``check()`` or ``sync_rng()`` raises for it; call ``check()`` every few
+    steps in long runs."""
 
     def __init__(self, swarm: ForageSwarm, configs: Sequence[WorldConfig], regen_words: int = 32,
                  device_rng: bool = False):
