@@ -299,6 +299,8 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
     bt->n_agents_total = (int32_t)agents_total;
     bt->field_words = words_total;
     bt->max_agents = max_n;
+    bt->min_agents = max_n;
+    for (int b = 0; b < n; ++b) bt->min_agents = std::min(bt->min_agents, reps[b].n_agents);
     bt->max_rays = max_r;
     bt->ray_total = ray_total;
     bt->n_tables = (int)table_off.size();
@@ -357,6 +359,11 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
         if (!rc) rc = check_hip(hipMalloc(&bt->grid_start, sizeof(int32_t) * (kWideGridMaxCells + 1)),
                                 "hipMalloc(grid cells)");
         if (!rc) rc = check_hip(hipMalloc(&bt->grid_param, 64), "hipMalloc(grid params)");
+        if (!rc && max_r <= 2048) rc = check_hip(hipMalloc(&bt->chain_fields,
+                                          sizeof(uint32_t) * (size_t)max_n * (size_t)((max_r + 31) / 32)),
+                                "hipMalloc(chain fields)");
+        if (!rc && max_r <= 2048)
+            rc = check_hip(hipMalloc(&bt->chain_u, sizeof(float) * 2 * (size_t)max_n), "hipMalloc(chain u)");
     }
     if (!rc) {
         bt->tab_sin = bt->tab_cos + ray_total;
@@ -402,6 +409,8 @@ int abm_vf_batch_destroy(abm_vf_batch* b) {
     if (b->wide_out) (void)hipFree(b->wide_out);
     if (b->grid_sorted) (void)hipFree(b->grid_sorted);
     if (b->grid_start) (void)hipFree(b->grid_start);
+    if (b->chain_fields) (void)hipFree(b->chain_fields);
+    if (b->chain_u) (void)hipFree(b->chain_u);
     if (b->grid_param) (void)hipFree(b->grid_param);
     for (GroupPlan& gp : b->gplan) {
         if (gp.groups) (void)hipFree(gp.groups);

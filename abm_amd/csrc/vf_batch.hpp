@@ -121,6 +121,9 @@ struct abm_vf_batch {
     float4* grid_sorted = nullptr;  // wide step: the replica's agents sorted by grid cell
     int32_t* grid_start = nullptr;  // wide step: first sorted agent of each cell (+ end)
     void* grid_param = nullptr;     // wide step: the grid of the current replica (WideGrid)
+    uint32_t* chain_fields = nullptr;   // chain step: every agent's field (max_agents x words)
+    void* chain_u = nullptr;        // chain step: start heading vectors (float2 per agent)
+    int32_t min_agents = 0;         // of the smallest replica
     abm::GroupPlan gplan[3];        // G = 1, 2, 4
     // host copies
     abm::VFRep* reps_host = nullptr;

@@ -327,14 +327,18 @@ struct WideSmem {
     int32_t* cand;      // kWideWaves lists of tile-local candidates
 };
 
-template <int RPL>
+// START: the chain step's first pass instead (see vf_step_f32_chain_kernel):
+// agent k's pairs with the agents after it in id order only, its field written
+// to start_fields[k W ..] and its heading vector to start_u[k]; no move.
+template <int RPL, bool START>
 __global__ void __launch_bounds__(64 * kWideWaves)
 vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
                         abm_vf_state_f32 st, float4* __restrict__ out_pose,
                         uint32_t* __restrict__ bits_out, float* __restrict__ diag, uint32_t flags,
                         unsigned long long* __restrict__ counters, float cull_c,
                         const WideGrid* __restrict__ grid, const float4* __restrict__ sorted,
-                        const int32_t* __restrict__ cell_start) {
+                        const int32_t* __restrict__ cell_start, uint32_t* __restrict__ start_fields,
+                        float2* __restrict__ start_u) {
     extern __shared__ __align__(16) double smem_d[];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -360,7 +364,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     uint32_t* F = S.F + wave * maxW;
     int32_t* cand = S.cand + wave * kWideTile;
     const int tab_off = rp->table_offset;
-    for (int e = tid; e < RPL * 64; e += 64 * kWideWaves) {
+    for (int e = tid; e < (START ? 0 : RPL * 64); e += 64 * kWideWaves) {
         const int r = (e % 64) * RPL + e / 64;
         RayTab32 z = tab32[tab_off + (r < C.R ? r : 0)];
         if (r >= C.R) z.wcos = z.wsin = z.cos = z.sin = 0.f;
@@ -402,7 +406,8 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
                     pos += 64;
                     nxt = sorted[pos + lane < end ? pos + lane : 0];
                     bool cnd = false;
-                    if (valid && __float_as_int(o.w) != k && !(o.x == fx && o.y == fy)) {
+                    const int j = __float_as_int(o.w);
+                    if (valid && (START ? j > k : j != k) && !(o.x == fx && o.y == fy)) {
                         float dx = (o.x - fx) + (o.z - fr), dy = (o.y - fy) + (o.z - fr);
                         const float ex = fabsf(dx) - C.hw, ey = fabsf(dy) - C.hh;
                         const bool near_wrap = fabsf(ex) < C.tguard || fabsf(ey) < C.tguard;
@@ -432,7 +437,9 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
         }
     }
 
-    for (int t0 = 0; t0 < (use_grid ? 0 : C.N); t0 += kWideTile) {
+    // (START: the tiles before the block's first agent hold no pair)
+    const int t_first = START ? (blockIdx.x * kWideWaves / kWideTile) * kWideTile : 0;
+    for (int t0 = t_first; t0 < (use_grid ? 0 : C.N); t0 += kWideTile) {
         __syncthreads();
         if (t0 + tid < C.N) {
             const int gj = C.a0 + t0 + tid;
@@ -445,7 +452,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
         for (int sub = 0; sub < kWideTile; sub += 64) {
             const int jt = sub + lane;
             bool c = false, culled = false;
-            if (active && jt < tn && t0 + jt != k) {
+            if (active && jt < tn && (START ? t0 + jt > k : t0 + jt != k)) {
                 const float4 o = S.tile[jt];
                 if (!(o.x == fx && o.y == fy)) {
                     if (count) ++n_pairs;
@@ -501,6 +508,19 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
         }
     }
     if (!use_grid) __syncthreads();
+    if (START) {
+        if (active) {
+            uint32_t* dst = start_fields + (size_t)k * C.W;
+            for (int w = lane; w < C.W; w += 64) dst[w] = F[w];
+            if (lane == 0) start_u[k] = make_float2(u1x, u1y);
+        }
+        if (count) {
+            if (n_pairs) atomicAdd(&counters[0], n_pairs);
+            if (n_recheck) atomicAdd(&counters[1], n_recheck);
+            if (n_disagree) atomicAdd(&counters[2], n_disagree);
+        }
+        return;
+    }
     if (!active) {
         if (count) {
             if (n_pairs) atomicAdd(&counters[0], n_pairs);
@@ -704,7 +724,7 @@ template <int RPL>
 int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                 uint32_t flags, hipStream_t s) {
     const size_t lds = lds_bytes_wide(RPL);
-    auto kern = vf_step_f32_wide_kernel<RPL>;
+    auto kern = vf_step_f32_wide_kernel<RPL, false>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -723,7 +743,7 @@ int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, flo
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWideWaves), lds, s, b->reps_dev + rep,
                            b->tab32, *st, b->wide_out, bits, diag, flags, b->counters, cull_c,
                            (const WideGrid*)grid, (const float4*)b->grid_sorted,
-                           (const int32_t*)b->grid_start);
+                           (const int32_t*)b->grid_start, nullptr, nullptr);
         if (int rc = check_hip(hipGetLastError(), "vf_step_f32_wide_kernel launch")) return rc;
         hipLaunchKernelGGL(copy_back_kernel, dim3((h.n_agents + 255) / 256), dim3(256), 0, s,
                            (const float4*)b->wide_out, *st, h.agent_offset, h.n_agents);
@@ -1025,11 +1045,12 @@ constexpr int kPushThreads = 64 * kPushWaves;
 // One pair of the push kernel: focal f (start pose, heading vector u), object
 // o; the float32 decision with the float64 one inside the guard bands (and
 // every pair in verify mode), rastered into the focal's field Fm.
-__device__ inline void push_pair(uint32_t* Fm, float2 u, float4 f, float4 o, const VFRep* rp,
-                                 const RepConst& C, bool count, bool verify, unsigned long long& n_pairs,
-                                 unsigned long long& n_recheck, unsigned long long& n_disagree) {
-    if (o.x == f.x && o.y == f.y) return;   // coincident (vf_supcalc.py:57)
-    PairF32 pr = fast_pair(u.x, u.y, f.x, f.y, f.z, o.x, o.y, o.z, C);
+// The decision alone: whether the pair draws, and its span in pr.start/end.
+__device__ inline bool decide_pair(PairF32& pr, float2 u, float4 f, float4 o, const VFRep* rp,
+                                   const RepConst& C, bool count, bool verify, unsigned long long& n_pairs,
+                                   unsigned long long& n_recheck, unsigned long long& n_disagree) {
+    if (o.x == f.x && o.y == f.y) return false;   // coincident (vf_supcalc.py:57)
+    pr = fast_pair(u.x, u.y, f.x, f.y, f.z, o.x, o.y, o.z, C);
     if (count) ++n_pairs;
     if (pr.tie && tie_in_place(pr, f.x, f.y, f.w, f.z, o.x, o.y, o.z, rp, C))   // rare
         if (count) ++n_recheck;
@@ -1042,7 +1063,15 @@ __device__ inline void push_pair(uint32_t* Fm, float2 u, float4 f, float4 o, con
             pr.start = es; pr.end = ee; pr.drawn = ed != 0;
         }
     }
-    if (pr.drawn) raster(Fm, pr.start, pr.end, C.R);
+    return pr.drawn;
+}
+
+__device__ inline void push_pair(uint32_t* Fm, float2 u, float4 f, float4 o, const VFRep* rp,
+                                 const RepConst& C, bool count, bool verify, unsigned long long& n_pairs,
+                                 unsigned long long& n_recheck, unsigned long long& n_disagree) {
+    PairF32 pr;
+    if (decide_pair(pr, u, f, o, rp, C, count, verify, n_pairs, n_recheck, n_disagree))
+        raster(Fm, pr.start, pr.end, C.R);
 }
 
 // Whether a pair lies beyond the visibility cut-off d > size cot(2 pi / R)
@@ -1338,6 +1367,417 @@ int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
     return launch_push<32>(b, st, bits, diag, flags, s, order, n_rep, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Reference (id) order for flocks above kWideThreshold agents, the chain
+// step.  Agent m's field is the OR of its pairs with every other agent j at
+// j's new pose for j < m and its start pose for j > m (m itself at its start
+// pose).  The pairs are decided as soon as both poses exist, mostly off the
+// id-order chain and over the whole GPU:
+//   1. wide_grid_kernel: the grid of the start poses (torus arenas whose
+//      cells can be at least the visibility cut-off wide).
+//   2. vf_step_f32_wide_kernel<RPL, true>: each agent's field with the agents
+//      after it, all at their start poses, into a global buffer of fields,
+//      and its start heading vector (one wave per agent).
+//   3. For each segment of kChainSeg agents in id order:
+//      a. vf_step_f32_segment_kernel (one workgroup of 512 threads) moves the
+//         segment's agents in id order, the segment's fields in LDS (their
+//         pairs with every earlier agent already ORed in).  Two barriers per
+//         agent k: A. every thread sums two 16-wave-kernel threads' rays of
+//         k's field (the same rays, reductions and order as that kernel, so
+//         the two give the same fields and poses bit for bit); B. wave 0
+//         moves agent k and decides the pair (k + 1, k) at k's new pose,
+//         waves 1..7 decide agent k - 1's new pose in the fields of the
+//         segment's agents after k (the cheap float32 cut-off test, the rest
+//         queued and decided 64 at a time).
+//      b. vf_step_f32_chain_push_kernel (the whole GPU) decides the segment's
+//         new poses in the global fields of every later agent: one wave per
+//         (moved agent, grid cell of its 3 x 3 neighbourhood), or per range
+//         of later agents without a grid (and with counters or verify mode).
+// Kernel boundaries order the global fields between the launches.
+constexpr int kChainThreads = 512;
+constexpr int kChainWaves = kChainThreads / 64;
+constexpr int kChainHelpers = kChainWaves - 1;
+constexpr int kChainSumWaves = 16;    // the 16-wave kernel's waves, summed in its order
+constexpr int kChainSeg = 256;        // agents per segment
+constexpr int kChainQueue = 128;      // queued agents per helper wave
+constexpr int kChainPushWaves = 4;    // waves per workgroup of the push kernel
+constexpr int kChainScanRange = 512;  // later agents per wave of the push kernel without a grid
+static_assert(kChainThreads * 2 == kBigThreads, "two 16-wave-kernel threads per thread");
+
+// Barrier of the segment kernel: this wave's LDS operations complete, then
+// s_barrier (the kernel shares no global memory between its waves).
+__device__ inline void chain_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// One thread's rays [rs, rs + rn) of the field F: the sums of w cos, w sin
+// (V) and cos, sin (the edges of V), as the 16-wave kernel's thread.
+template <int RPL>
+__device__ inline void chain_ray_sums(const uint32_t* F, const RayTab32* T, int rs, int rn, bool case1,
+                                      const RepConst& C, float& bc, float& bs, float& ec, float& es) {
+    bc = bs = ec = es = 0.f;
+    if (rn <= 0) return;
+    const int w0 = rs >> 5;
+    const uint32_t lo = F[w0];
+    const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? F[w0 + 1] : 0u;
+    const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
+    const uint32_t m = low_mask(rn);
+    const uint32_t cur = (uint32_t)v64 & m;
+    uint32_t spike;
+    if (case1) {
+        const uint32_t prev = field_bit(F, rs == 0 ? C.R - 1 : rs - 1);
+        spike = (cur ^ ((cur << 1) | prev)) & m;
+    } else {
+        const int nx = rs + rn;
+        const uint32_t next = field_bit(F, nx >= C.R ? 0 : nx);
+        spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const bool vb = (cur >> i) & 1u;
+        const bool sb = (spike >> i) & 1u;
+        bc += vb ? T[i].wcos : 0.f;
+        bs += vb ? T[i].wsin : 0.f;
+        ec += sb ? T[i].cos : 0.f;
+        es += sb ? T[i].sin : 0.f;
+    }
+}
+
+// The agents [s0, s0 + n) of one replica, n <= kChainSeg.
+template <int RPL>
+__global__ void __launch_bounds__(kChainThreads)
+vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
+                           abm_vf_state_f32 st, const uint32_t* __restrict__ fields,
+                           const float2* __restrict__ ustart, uint32_t* __restrict__ bits_out,
+                           float* __restrict__ diag, uint32_t flags, unsigned long long* __restrict__ counters,
+                           float cull_c, int s0, int n) {
+    __shared__ uint32_t rows[kChainSeg][64];
+    __shared__ float4 srec[kChainSeg];     // start poses (x, y, radius, theta)
+    __shared__ float2 su[kChainSeg];       // start heading vectors
+    __shared__ double red[4 * kChainSumWaves];
+    __shared__ float4 newpose[2];
+    __shared__ uint32_t queue[kChainWaves][kChainQueue];
+    const RepConst C = rep_const(rp);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    for (int i = tid; i < n; i += kChainThreads) {
+        const int g = C.a0 + s0 + i;
+        srec[i] = make_float4(st.x[g], st.y[g], (float)st.radius[g], st.theta[g]);
+        su[i] = ustart[s0 + i];
+    }
+    for (int w = tid; w < n * 64; w += kChainThreads) {
+        const int i = w >> 6, c = w & 63;
+        rows[i][c] = c < C.W ? fields[(size_t)(s0 + i) * C.W + c] : 0u;
+    }
+    // this thread's rays: those of the 16-wave kernel's threads tid and
+    // tid + 512, and their table entries
+    const int rs0 = tid * RPL, rs1 = (tid + kChainThreads) * RPL;
+    const int rn0 = rs0 >= C.R ? 0 : (C.R - rs0 < RPL ? C.R - rs0 : RPL);
+    const int rn1 = rs1 >= C.R ? 0 : (C.R - rs1 < RPL ? C.R - rs1 : RPL);
+    RayTab32 T0[RPL], T1[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int r0 = rs0 + i, r1 = rs1 + i;
+        if (r0 < C.R) T0[i] = tab32[rp->table_offset + r0];
+        else T0[i].wcos = T0[i].wsin = T0[i].cos = T0[i].sin = 0.f;
+        if (r1 < C.R) T1[i] = tab32[rp->table_offset + r1];
+        else T1[i].wcos = T1[i].wsin = T1[i].cos = T1[i].sin = 0.f;
+    }
+    const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
+    // wave 0: agent k's speed and parameters, loaded one agent ahead
+    float vel_k = 0.f, alp0_k = a0d, bet0_k = b0d, v0_k = v0d;
+    if (wave == 0) {
+        const int g = C.a0 + s0;
+        vel_k = st.v[g];
+        if (st.alp0) alp0_k = st.alp0[g];
+        if (st.bet0) bet0_k = st.bet0[g];
+        if (st.v0) v0_k = st.v0[g];
+    }
+    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    uint32_t* wq = queue[wave];
+    __syncthreads();
+    if (wave == 0) __builtin_amdgcn_s_setprio(3);   // the chain's wave issues first
+    for (int kk = 0; kk < n; ++kk) {
+        const int k = s0 + kk;
+        // ---- A: agent k's sums (every thread); agent k + 1's speed (wave 0)
+        float vel_n = 0.f, alp0_n = a0d, bet0_n = b0d, v0_n = v0d;
+        if (wave == 0 && kk + 1 < n) {
+            const int g = C.a0 + k + 1;
+            vel_n = st.v[g];
+            if (st.alp0) alp0_n = st.alp0[g];
+            if (st.bet0) bet0_n = st.bet0[g];
+            if (st.v0) v0_n = st.v0[g];
+        }
+        const uint32_t* Fk = rows[kk];
+        {
+            const bool case1 = (Fk[0] & 1u) == 1u && field_bit(Fk, C.R - 1) == 0u;
+            float bc, bs, ec, es;
+            chain_ray_sums<RPL>(Fk, T0, rs0, rn0, case1, C, bc, bs, ec, es);
+            const double Bc0 = wave_total(bc), Bs0 = wave_total(bs), Ec0 = wave_total(ec), Es0 = wave_total(es);
+            chain_ray_sums<RPL>(Fk, T1, rs1, rn1, case1, C, bc, bs, ec, es);
+            const double Bc1 = wave_total(bc), Bs1 = wave_total(bs), Ec1 = wave_total(ec), Es1 = wave_total(es);
+            if (lane == 0) {
+                double* r0 = red + 4 * wave;
+                double* r1 = red + 4 * (wave + kChainWaves);
+                r0[0] = Bc0; r0[1] = Bs0; r0[2] = Ec0; r0[3] = Es0;
+                r1[0] = Bc1; r1[1] = Bs1; r1[2] = Ec1; r1[3] = Es1;
+            }
+        }
+        if (bits_out && tid < C.W) bits_out[C.field_offset + (int64_t)k * C.W + tid] = Fk[tid];
+        chain_barrier();
+        // ---- B
+        if (wave == 0) {
+            const int gk = C.a0 + k;
+            double Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
+            for (int w = 0; w < kChainSumWaves; ++w) {
+                Bc += red[4 * w]; Bs += red[4 * w + 1]; Ec += red[4 * w + 2]; Es += red[4 * w + 3];
+            }
+            const float4 pk = srec[kk];
+            const float fx = pk.x, fy = pk.y, fr = pk.z, fth = pk.w;
+            const float alp0 = alp0_k, bet0 = bet0_k, v0 = v0_k;
+            const double Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+            const double Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+            float vel, o, dv, dpsi;
+            steer(C, fth, vel_k, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
+            float so, co;
+            sincos_f32(o, &so, &co);
+            float nx, ny;
+            euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
+            const float4 moved = make_float4(nx, ny, fr, o);
+            if (lane == 0) {
+                newpose[kk & 1] = moved;
+                st.x[gk] = nx;
+                st.y[gk] = ny;
+                st.theta[gk] = o;
+                st.v[gk] = vel;
+                if (want_diag) {
+                    float* dd = diag + 6 * (int64_t)gk;
+                    dd[0] = dv; dd[1] = dpsi;
+                    dd[2] = C.phi_ok ? (float)((double)alp0 * -Bc) : 0.f;
+                    dd[3] = C.phi_ok ? (float)((double)(alp0 * C.alp1) * Ec) : 0.f;
+                    dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
+                    dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
+                }
+                if (kk + 1 < n) {
+                    const float4 pn = srec[kk + 1];
+                    if (push_culled(pn, moved, C, cull_c))
+                        push_culled_check(pn, moved, rp, count, verify, n_pairs, n_disagree);
+                    else
+                        push_pair(rows[kk + 1], su[kk + 1], pn, moved, rp, C, count, verify, n_pairs, n_recheck,
+                                  n_disagree);
+                }
+            }
+            vel_k = vel_n; alp0_k = alp0_n; bet0_k = bet0_n; v0_k = v0_n;
+        } else if (kk >= 1) {
+            // agent k - 1's new pose into the fields of the segment's agents
+            // after k: the cut-off test 64 at a time, the rest queued in a
+            // ring of 128 and decided 64 at a time
+            const float4 o = newpose[(kk - 1) & 1];
+            int head = 0, tail = 0;   // wave-uniform ring positions
+            for (int mb = kk + 1 + (wave - 1) * 64; mb < n; mb += kChainHelpers * 64) {
+                const int m = mb + lane;
+                bool keep = false;
+                if (m < n) {
+                    const float4 f = srec[m];
+                    if (!(o.x == f.x && o.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
+                        keep = !push_culled(f, o, C, cull_c);
+                        if (!keep) push_culled_check(f, o, rp, count, verify, n_pairs, n_disagree);
+                    }
+                }
+                const uint64_t mk = __ballot(keep);
+                if (keep) wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kChainQueue - 1)] = (uint32_t)m;
+                tail += (int)__popcll(mk);
+                wave_lds_sync();
+                if (tail - head >= 64) {
+                    const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
+                    push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
+                    head += 64;
+                    wave_lds_sync();
+                }
+            }
+            if (lane < tail - head) {
+                const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
+                push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
+            }
+        }
+        chain_barrier();
+    }
+    if (count) {
+        if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+    }
+}
+
+// The new poses of the agents [s0, s0 + n) (just moved) into the global
+// fields of the agents from s1 = s0 + n on (at their start poses).  Wave
+// (j, c): moved agent s0 + j and chunk c of the later agents -- grid cell c
+// of the 3 x 3 around its new centre, or the range [s1 + c R, s1 + (c + 1) R)
+// (R = kChainScanRange) without a grid.
+__global__ void __launch_bounds__(64 * kChainPushWaves)
+vf_step_f32_chain_push_kernel(const VFRep* __restrict__ rp, abm_vf_state_f32 st, uint32_t* __restrict__ fields,
+                              const float2* __restrict__ ustart, uint32_t flags,
+                              unsigned long long* __restrict__ counters, float cull_c,
+                              const WideGrid* __restrict__ grid, const float4* __restrict__ sorted,
+                              const int32_t* __restrict__ cell_start, int s0, int n, int n_chunks) {
+    __shared__ uint32_t queue[kChainPushWaves][kChainQueue];
+    const RepConst C = rep_const(rp);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const WideGrid G = *grid;
+    const bool use_grid = G.ok && !count && !verify;
+    const int item = blockIdx.x * kChainPushWaves + wave;
+    const int j = item / n_chunks, c = item - j * n_chunks;
+    const int s1 = s0 + n;
+    // whole waves only (no barrier below): past the moved agents, or past the
+    // nine cells of the grid
+    if (j >= n || (use_grid && c >= 9)) return;
+    const int gj = C.a0 + s0 + j;
+    const float4 o = make_float4(st.x[gj], st.y[gj], (float)st.radius[gj], st.theta[gj]);
+    uint32_t* wq = queue[wave];
+    unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
+    int beg, end;
+    if (use_grid) {
+        const int oc = grid_cell(G, o.x + o.z, o.y + o.z);
+        const int oiy = oc / G.nx, oix = oc - oiy * G.nx;
+        const int ty = c / 3, tx = c - 3 * ty;
+        int cy = oiy + ty - 1, cx = oix + tx - 1;   // the 3 x 3 cells, wrapped
+        cy = cy < 0 ? G.ny - 1 : (cy >= G.ny ? 0 : cy);
+        cx = cx < 0 ? G.nx - 1 : (cx >= G.nx ? 0 : cx);
+        const int cell = cy * G.nx + cx;
+        beg = cell_start[cell];
+        end = cell_start[cell + 1];
+    } else {
+        beg = s1 + c * kChainScanRange;
+        end = min(C.N, beg + kChainScanRange);
+    }
+    // focal agent m (record i), at its start pose
+    auto focal = [&](int i, int& m, bool with_theta) -> float4 {
+        if (use_grid) {
+            const float4 s = sorted[i];
+            m = __float_as_int(s.w);
+            return make_float4(s.x, s.y, s.z, with_theta ? st.theta[C.a0 + m] : 0.f);
+        }
+        m = i;
+        const int g = C.a0 + i;
+        return make_float4(st.x[g], st.y[g], (float)st.radius[g], with_theta ? st.theta[g] : 0.f);
+    };
+    auto decide = [&](int head, int tail) {
+        if (head + lane < tail) {
+            int m;
+            const float4 f = focal((int)wq[(head + lane) & (kChainQueue - 1)], m, true);
+            push_pair(fields + (size_t)m * C.W, ustart[m], f, o, rp, C, count, verify, n_pairs, n_recheck,
+                      n_disagree);
+        }
+    };
+    int head = 0, tail = 0;   // wave-uniform ring positions
+    for (int b = beg; b < end; b += 64) {
+        const int i = b + lane;
+        bool keep = false;
+        if (i < end) {
+            int m;
+            const float4 f = focal(i, m, verify);
+            if (m >= s1 && !(o.x == f.x && o.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
+                keep = !push_culled(f, o, C, cull_c);
+                if (!keep) push_culled_check(f, o, rp, count, verify, n_pairs, n_disagree);
+            }
+        }
+        const uint64_t mk = __ballot(keep);
+        if (keep) wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kChainQueue - 1)] = (uint32_t)i;
+        tail += (int)__popcll(mk);
+        wave_lds_sync();
+        if (tail - head >= 64) {
+            decide(head, tail);
+            head += 64;
+            wave_lds_sync();
+        }
+    }
+    decide(head, tail);
+    if (count) {
+        if (n_pairs) atomicAdd(&counters[0], n_pairs);
+        if (n_recheck) atomicAdd(&counters[1], n_recheck);
+        if (n_disagree) atomicAdd(&counters[2], n_disagree);
+    }
+}
+
+template <int RPLW>
+int launch_chain_start(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t flags, hipStream_t s,
+                       int rep, float cull_c, int n_agents) {
+    const size_t lds = lds_bytes_wide(RPLW);
+    auto kern = vf_step_f32_wide_kernel<RPLW, true>;
+    if (lds > 65536) {
+        if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                               "hipFuncSetAttribute"))
+            return rc;
+    }
+    const int blocks = (n_agents + kWideWaves - 1) / kWideWaves;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWideWaves), lds, s, b->reps_dev + rep, b->tab32, *st,
+                       (float4*)nullptr, (uint32_t*)nullptr, (float*)nullptr, flags, b->counters, cull_c,
+                       (const WideGrid*)b->grid_param, (const float4*)b->grid_sorted,
+                       (const int32_t*)b->grid_start, b->chain_fields, (float2*)b->chain_u);
+    return check_hip(hipGetLastError(), "vf_step_f32_wide_kernel (start pairs) launch");
+}
+
+// Whether the chain step runs the batch: every replica above kWideThreshold
+// agents (a single large flock), the reference order, R up to 2048.
+bool chain_fits(const abm_vf_batch* b) {
+    return b->chain_fields != nullptr && b->chain_u != nullptr && b->n_replicas > 0 &&
+           b->min_agents > kWideThreshold && b->max_rays <= 2 * kBigThreads;
+}
+
+int launch_chain(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag, uint32_t flags,
+                 hipStream_t s) {
+    const bool counting = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0;
+    for (int rep = 0; rep < b->n_replicas; ++rep) {
+        const VFRep& h = b->reps_host[rep];
+        const int N = h.n_agents;
+        const float cull_c = (float)h.cull_c;
+        hipLaunchKernelGGL(wide_grid_kernel, dim3(1), dim3(1024), 0, s, b->reps_dev + rep, *st, cull_c,
+                           b->grid_sorted, b->grid_start, (WideGrid*)b->grid_param);
+        if (int rc = check_hip(hipGetLastError(), "wide_grid_kernel launch")) return rc;
+        const int need = (h.n_rays + 63) / 64;
+        int rc;
+        if (need <= 4) rc = launch_chain_start<4>(b, st, flags, s, rep, cull_c, N);
+        else if (need <= 8) rc = launch_chain_start<8>(b, st, flags, s, rep, cull_c, N);
+        else if (need <= 16) rc = launch_chain_start<16>(b, st, flags, s, rep, cull_c, N);
+        else rc = launch_chain_start<32>(b, st, flags, s, rep, cull_c, N);
+        if (rc) return rc;
+        for (int s0 = 0; s0 < N; s0 += kChainSeg) {
+            const int n = std::min(kChainSeg, N - s0);
+            if (h.n_rays <= kBigThreads)   // rays per 16-wave-kernel thread, as that kernel
+                hipLaunchKernelGGL(vf_step_f32_segment_kernel<1>, dim3(1), dim3(kChainThreads), 0, s,
+                                   b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,
+                                   (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
+            else
+                hipLaunchKernelGGL(vf_step_f32_segment_kernel<2>, dim3(1), dim3(kChainThreads), 0, s,
+                                   b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,
+                                   (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
+            if (int rc2 = check_hip(hipGetLastError(), "vf_step_f32_segment_kernel launch")) return rc2;
+            const int s1 = s0 + n;
+            if (s1 >= N) break;
+            // chunks per moved agent: enough for the nine cells of a grid and
+            // for the ranges of the later agents (the device's grid decides
+            // which apply; the other waves return at once)
+            const int n_ranges = (N - s1 + kChainScanRange - 1) / kChainScanRange;
+            const int n_chunks = counting ? n_ranges : std::max(9, n_ranges);
+            const int items = n * n_chunks;
+            hipLaunchKernelGGL(vf_step_f32_chain_push_kernel, dim3((items + kChainPushWaves - 1) / kChainPushWaves),
+                               dim3(64 * kChainPushWaves), 0, s, b->reps_dev + rep, *st, b->chain_fields,
+                               (const float2*)b->chain_u, flags, b->counters, cull_c, (const WideGrid*)b->grid_param,
+                               (const float4*)b->grid_sorted, (const int32_t*)b->grid_start, s0, n, n_chunks);
+            if (int rc2 = check_hip(hipGetLastError(), "vf_step_f32_chain_push_kernel launch")) return rc2;
+        }
+    }
+    b->last_kernels |= ABM_KERNEL_CHAIN;
+    return ABM_OK;
+}
+
 size_t lds_bytes_f32(int maxN, int maxR, int nw, int rpl, bool sync_order) {
     const int maxW = (maxR + 31) >> 5;
     size_t b = sizeof(RayTab32) * (size_t)rpl * 64 * nw + sizeof(float4) * (size_t)maxN * (sync_order ? 3 : 2) +
@@ -1441,7 +1881,12 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32: ABM_KAHAN_SUMS is implemented by the "
                          "grouped kernel only (replicas of at most %d agents)", kGroupMaxAgents);
     if (b->max_agents > kWideThreshold) {
-        if (!sync_order)   // reference order: one workgroup per replica, every pose in LDS
+        // reference order: the chain step for single large flocks (unless the
+        // 16-wave kernel is forced), else one workgroup per replica with every
+        // pose in LDS
+        if (!sync_order && b->waves_per_group != 16 && chain_fits(b))
+            return launch_chain(b, st, bits, diag, flags, s);
+        if (!sync_order)
             return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
                                   b->max_rays);
         if (!b->wide_out)

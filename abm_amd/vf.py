@@ -236,7 +236,7 @@ class VFSwarm:
         """Names of the kernels the last step launched."""
         m = C.c_uint32()
         L.check(self.batch.lib.abm_vf_last_kernels(self.batch.handle, C.byref(m)))
-        names = ("grouped", "per_replica", "wide", "big", "push", "f64")
+        names = ("grouped", "per_replica", "wide", "big", "push", "f64", "chain")
         return {n for i, n in enumerate(names) if m.value >> i & 1}
 
     def set_grouping(self, replicas_per_wave: int):

@@ -189,6 +189,7 @@ int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
 #define ABM_KERNEL_BIG 8u
 #define ABM_KERNEL_PUSH 16u
 #define ABM_KERNEL_F64 32u
+#define ABM_KERNEL_CHAIN 64u
 int abm_vf_last_kernels(const abm_vf_batch* batch, uint32_t* mask);
 /* replicas_per_wave: 1, 2 or 4 = grouped kernel with that many replicas per
  * wave (ABM_ERR_UNSUPPORTED when the batch does not qualify), -1 = per-replica

@@ -1,0 +1,102 @@
+"""The chain step: single flocks above 2048 agents in the reference (id) order
+(vf_sims.py:302 -> vf_agent.py:52-77).
+
+Each agent's pairs with the agents after it are decided at their start poses
+over the whole GPU first; one workgroup then moves the agents in id order,
+deciding each moved agent's new pose in the fields of the agents after it
+(through the grid of start poses) while the next agents move.  It sums the
+same rays per thread in the same order as the 16-wave kernel (every pose in
+LDS, each agent's pairs decided in its turn) and every decision is exact, so
+the two must agree bit for bit: fields, poses, speeds and the integrals."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+PI = np.pi
+f32 = lambda a: np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def flock(N, R, side, seed, boundary="infinite", seam=False, radius=2, params=False):
+    from abm_amd.vf import VFConfig
+    rng = np.random.RandomState(seed)
+    if seam:   # half the agents within 4 px of the periodic seam in x
+        lo, hi = 25.0, 25.0 + side
+        x = np.where(rng.rand(N) < 0.5, rng.uniform(lo - 4, lo + 4, N), rng.uniform(hi - 4, hi + 4, N))
+    else:
+        x = rng.uniform(30, 30 + side, N)
+    y = rng.uniform(30, 30 + side, N)
+    th, v = rng.uniform(0, 2 * PI, N), rng.uniform(0, 1.5, N)
+    if radius == "mixed":
+        radius = rng.randint(1, 7, N)
+    extra = {}
+    if params:   # per-agent ALP0 / BET0 / V0
+        extra = dict(alp0=f32(rng.uniform(0.5, 1.5, N)), bet0=f32(rng.uniform(0.2, 0.8, N)),
+                     v0=f32(rng.uniform(0.5, 1.5, N)))
+    cfg = VFConfig(n_agents=N, n_rays=R, width=side, height=side, boundary=boundary, ALP0=1.0, BET0=0.5)
+    return cfg, (f32(x), f32(y), f32(th), f32(v)), radius, extra
+
+
+CASES = {
+    # C5 itself: grid path, two rays per thread
+    "c5": dict(N=8192, R=2048, side=4000.0, seed=11),
+    # agents crowded on the seam of a 3 x 3-cell torus, one ray per thread
+    "seam": dict(N=2304, R=1024, side=1500.0, seed=5, seam=True, radius=3),
+    # walls: no grid, every agent after k tested
+    "walls": dict(N=3000, R=512, side=2400.0, seed=8, boundary="walls", radius=5),
+    # a torus too small for three cells a side: no grid
+    "small_torus": dict(N=2500, R=2048, side=1200.0, seed=9),
+    # radii 1..6 (the grid's reach follows the largest) and per-agent parameters
+    "mixed": dict(N=4096, R=1536, side=3000.0, seed=12, radius="mixed", params=True),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_chain_matches_16_wave_kernel(gpu, case):
+    from abm_amd.vf import VFSwarm
+    cfg, st, radius, extra = flock(**CASES[case])
+    a = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    b = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    b.set_launch(16)
+    for step in range(3):
+        print(f"  chain {case} step {step}", flush=True)
+        a.step(order="sequential", fields=True, diag=True)
+        b.step(order="sequential", fields=True, diag=True)
+        assert a.last_kernels() == {"chain"} and b.last_kernels() == {"big"}
+        assert np.array_equal(a.fields_numpy(0), b.fields_numpy(0)), step
+        for p, q in zip(a.state_numpy(), b.state_numpy()):
+            assert np.array_equal(p, q), step
+        assert np.array_equal(a.diag_numpy(), b.diag_numpy()), step
+
+
+@pytest.mark.parametrize("case", ["c5", "seam", "walls"])
+def test_chain_verify_mode(gpu, case):
+    # counters and verify mode take the full scans: every pair is counted
+    # once, every float32 decision and every dropped pair agrees with float64,
+    # and the result equals the default path's
+    from abm_amd.vf import VFSwarm
+    cfg, st, radius, extra = flock(**CASES[case])
+    a = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    b = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    a.step(order="sequential", fields=True, verify=True)
+    b.step(order="sequential", fields=True)
+    assert a.last_kernels() == {"chain"}
+    pairs, _, disagree = a.fallback_counts()
+    N = cfg.n_agents
+    assert disagree == 0 and pairs == N * (N - 1)
+    assert np.array_equal(a.fields_numpy(0), b.fields_numpy(0))
+    for p, q in zip(a.state_numpy(), b.state_numpy()):
+        assert np.array_equal(p, q)
+
+
+def test_chain_sampled_agents_match_oracle(gpu):
+    # N = 2500 > 2048 on a 2400 px torus (grid path): sampled agents'
+    # fields and poses against the oracle on the poses each agent saw
+    from abm_amd.vf import VFSwarm
+    from test_gpu_vf_f32 import per_agent_check
+    cfg, st, radius, extra = flock(N=2500, R=2048, side=2400.0, seed=3)
+    sw = VFSwarm([cfg], *st, radius=radius, precision="f32")
+    sw.step(order="sequential", fields=True, diag=True)
+    assert sw.last_kernels() == {"chain"}
+    agents = sorted({0, 1, 2, 1249, 2497, 2498, 2499} |
+                    set(np.random.RandomState(1).choice(2500, 12, replace=False).tolist()))
+    per_agent_check(sw, 0, cfg, st, sw.state_numpy(), radius=2, agents=agents)
