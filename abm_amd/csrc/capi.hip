@@ -496,8 +496,22 @@ int abm_vf_last_kernels(const abm_vf_batch* b, uint32_t* mask) {
 
 int abm_vf_set_launch(abm_vf_batch* b, int32_t waves) {
     if (!b) return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: NULL batch");
-    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 16 && waves != 32)
-        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2, 4, 16 or 32");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 16 && waves != 32 && waves != 64)
+        return set_error(ABM_ERR_INVALID, "abm_vf_set_launch: waves_per_group must be 0, 1, 2, 4, 16, 32 or 64");
+    if (waves == 64) {   // the chain step for every replica: its buffers, if the batch has none yet
+        if (b->max_rays > 2048)
+            return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_set_launch: the chain step takes n_rays up to 2048");
+        const size_t n = (size_t)b->max_agents, words = (size_t)((b->max_rays + 31) / 32);
+        int rc = ABM_OK;
+        if (!b->grid_sorted) rc = check_hip(hipMalloc(&b->grid_sorted, sizeof(float4) * n), "hipMalloc(grid)");
+        if (!rc && !b->grid_start)
+            rc = check_hip(hipMalloc(&b->grid_start, sizeof(int32_t) * (kWideGridMaxCells + 1)), "hipMalloc(grid cells)");
+        if (!rc && !b->grid_param) rc = check_hip(hipMalloc(&b->grid_param, 64), "hipMalloc(grid params)");
+        if (!rc && !b->chain_fields)
+            rc = check_hip(hipMalloc(&b->chain_fields, sizeof(uint32_t) * n * words), "hipMalloc(chain fields)");
+        if (!rc && !b->chain_u) rc = check_hip(hipMalloc(&b->chain_u, sizeof(float) * 2 * n), "hipMalloc(chain u)");
+        if (rc) return rc;
+    }
     b->waves_per_group = waves;
     b->group_g = waves > 0 ? -1 : 0;
     return ABM_OK;

@@ -1042,6 +1042,60 @@ int launch_big_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, 
 // from LDS, the loads of the sums waited on the other waves' LDS traffic.
 constexpr int kPushThreads = 64 * kPushWaves;
 
+// The one-wave sums of a field Fk, as the one-wave per-replica kernel takes
+// them: lane l sums its rays [l RPL, (l + 1) RPL) (table entries T) in float32,
+// the wave totals are float64.  Without diagnostics the two integrals are
+// totalled directly (Sv, Sp); with them the four parts (Bc, Bs, Ec, Es) and
+// the integrals from those.  Shared by the push kernel and the chain step's
+// segment kernel, which thereby give the same results bit for bit.
+template <int RPL>
+__device__ __forceinline__ void one_wave_sums(const uint32_t* Fk, const RayTab32* T, int lane, const RepConst& C,
+                                              bool want_diag, float alp0, float bet0, double& Sv, double& Sp,
+                                              double& Bc, double& Bs, double& Ec, double& Es) {
+    const int rs = lane * RPL;
+    const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
+    const bool case1 = (Fk[0] & 1u) == 1u && field_bit(Fk, C.R - 1) == 0u;
+    float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
+    {
+        const int w0 = rs >> 5;
+        const uint32_t lo = w0 < C.W ? Fk[w0] : 0u;
+        const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? Fk[w0 + 1] : 0u;
+        const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
+        const uint32_t m = low_mask(rn);
+        const uint32_t cur = (uint32_t)v64 & m;
+        uint32_t spike = 0u;
+        if (rn > 0) {
+            if (case1) {
+                const uint32_t prev = field_bit(Fk, rs == 0 ? C.R - 1 : rs - 1);
+                spike = (cur ^ ((cur << 1) | prev)) & m;
+            } else {
+                const int nx = rs + rn;
+                const uint32_t next = field_bit(Fk, nx >= C.R ? 0 : nx);
+                spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const bool vb = (cur >> i) & 1u;
+            const bool sb = (spike >> i) & 1u;
+            bc += vb ? T[i].wcos : 0.f;
+            bs += vb ? T[i].wsin : 0.f;
+            ec += sb ? T[i].cos : 0.f;
+            es += sb ? T[i].sin : 0.f;
+        }
+    }
+    Bc = Bs = Ec = Es = 0.0;
+    if (!want_diag) {
+        Sv = wave_total(alp0 * (-bc) + (alp0 * C.alp1) * ec);
+        Sp = wave_total(bet0 * (-bs) + (bet0 * C.bet1) * es);
+    } else {
+        Bc = wave_total(bc); Bs = wave_total(bs);
+        Ec = wave_total(ec); Es = wave_total(es);
+        Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
+        Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+    }
+}
+
 // One pair of the push kernel: focal f (start pose, heading vector u), object
 // o; the float32 decision with the float64 one inside the guard bands (and
 // every pair in verify mode), rastered into the focal's field Fm.
@@ -1189,57 +1243,17 @@ vf_step_f32_push_kernel(const VFRep* __restrict__ reps, const int32_t* __restric
         }
     }
     __syncthreads();
-    const int rs = lane * RPL;   // wave 0: this lane's rays
-    const int rn = rs >= C.R ? 0 : (C.R - rs < RPL ? C.R - rs : RPL);
     if (wave == 0) __builtin_amdgcn_s_setprio(3);   // the chain's wave issues first
     for (int k = 0; k < C.N; ++k) {
         if (wave == 0) {
             // ---- agent k's sums, as the one-wave per-replica kernel
             const uint32_t* Fk = F + (size_t)k * W1;
-            const bool case1 = (Fk[0] & 1u) == 1u && field_bit(Fk, C.R - 1) == 0u;
-            float bc = 0.f, bs = 0.f, ec = 0.f, es = 0.f;
-            {
-                const int w0 = rs >> 5;
-                const uint32_t lo = w0 < C.W ? Fk[w0] : 0u;
-                const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? Fk[w0 + 1] : 0u;
-                const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
-                const uint32_t m = low_mask(rn);
-                const uint32_t cur = (uint32_t)v64 & m;
-                uint32_t spike = 0u;
-                if (rn > 0) {
-                    if (case1) {
-                        const uint32_t prev = field_bit(Fk, rs == 0 ? C.R - 1 : rs - 1);
-                        spike = (cur ^ ((cur << 1) | prev)) & m;
-                    } else {
-                        const int nx = rs + rn;
-                        const uint32_t next = field_bit(Fk, nx >= C.R ? 0 : nx);
-                        spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < RPL; ++i) {
-                    const bool vb = (cur >> i) & 1u;
-                    const bool sb = (spike >> i) & 1u;
-                    bc += vb ? T[i].wcos : 0.f;
-                    bs += vb ? T[i].wsin : 0.f;
-                    ec += sb ? T[i].cos : 0.f;
-                    es += sb ? T[i].sin : 0.f;
-                }
-            }
             const float4 fp = P0[k];
             const float4 dk = dyn[k];
             const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
             const float alp0 = dk.y, bet0 = dk.z, v0 = dk.w;
-            double Sv, Sp, Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
-            if (!want_diag) {
-                Sv = wave_total(alp0 * (-bc) + (alp0 * C.alp1) * ec);
-                Sp = wave_total(bet0 * (-bs) + (bet0 * C.bet1) * es);
-            } else {
-                Bc = wave_total(bc); Bs = wave_total(bs);
-                Ec = wave_total(ec); Es = wave_total(es);
-                Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
-                Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
-            }
+            double Sv, Sp, Bc, Bs, Ec, Es;
+            one_wave_sums<RPL>(Fk, T, lane, C, want_diag, alp0, bet0, Sv, Sp, Bc, Bs, Ec, Es);
             if (bits_out) {
                 uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
                 for (int w = lane; w < C.W; w += 64) dst[w] = Fk[w];
@@ -1381,14 +1395,9 @@ int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
 //   3. For each segment of kChainSeg agents in id order:
 //      a. vf_step_f32_segment_kernel (one workgroup of 512 threads) moves the
 //         segment's agents in id order, the segment's fields in LDS (their
-//         pairs with every earlier agent already ORed in).  Two barriers per
-//         agent k: A. every thread sums two 16-wave-kernel threads' rays of
-//         k's field (the same rays, reductions and order as that kernel, so
-//         the two give the same fields and poses bit for bit); B. wave 0
-//         moves agent k and decides the pair (k + 1, k) at k's new pose,
-//         waves 1..7 decide agent k - 1's new pose in the fields of the
-//         segment's agents after k (the cheap float32 cut-off test, the rest
-//         queued and decided 64 at a time).
+//         pairs with every earlier agent already ORed in), summing and moving
+//         as the push kernel does, so the two give the same fields and poses
+//         bit for bit.
 //      b. vf_step_f32_chain_push_kernel (the whole GPU) decides the segment's
 //         new poses in the global fields of every later agent: one wave per
 //         (moved agent, grid cell of its 3 x 3 neighbourhood), or per range
@@ -1397,12 +1406,10 @@ int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
 constexpr int kChainThreads = 512;
 constexpr int kChainWaves = kChainThreads / 64;
 constexpr int kChainHelpers = kChainWaves - 1;
-constexpr int kChainSumWaves = 16;    // the 16-wave kernel's waves, summed in its order
 constexpr int kChainSeg = 256;        // agents per segment
 constexpr int kChainQueue = 128;      // queued agents per helper wave
 constexpr int kChainPushWaves = 4;    // waves per workgroup of the push kernel
 constexpr int kChainScanRange = 512;  // later agents per wave of the push kernel without a grid
-static_assert(kChainThreads * 2 == kBigThreads, "two 16-wave-kernel threads per thread");
 
 // Barrier of the segment kernel: this wave's LDS operations complete, then
 // s_barrier (the kernel shares no global memory between its waves).
@@ -1412,40 +1419,20 @@ __device__ inline void chain_barrier() {
     asm volatile("" ::: "memory");
 }
 
-// One thread's rays [rs, rs + rn) of the field F: the sums of w cos, w sin
-// (V) and cos, sin (the edges of V), as the 16-wave kernel's thread.
-template <int RPL>
-__device__ inline void chain_ray_sums(const uint32_t* F, const RayTab32* T, int rs, int rn, bool case1,
-                                      const RepConst& C, float& bc, float& bs, float& ec, float& es) {
-    bc = bs = ec = es = 0.f;
-    if (rn <= 0) return;
-    const int w0 = rs >> 5;
-    const uint32_t lo = F[w0];
-    const uint32_t hi = (RPL > 1 && w0 + 1 < C.W) ? F[w0 + 1] : 0u;
-    const uint64_t v64 = (((uint64_t)hi << 32) | lo) >> (rs & 31);
-    const uint32_t m = low_mask(rn);
-    const uint32_t cur = (uint32_t)v64 & m;
-    uint32_t spike;
-    if (case1) {
-        const uint32_t prev = field_bit(F, rs == 0 ? C.R - 1 : rs - 1);
-        spike = (cur ^ ((cur << 1) | prev)) & m;
-    } else {
-        const int nx = rs + rn;
-        const uint32_t next = field_bit(F, nx >= C.R ? 0 : nx);
-        spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
-    }
-#pragma unroll
-    for (int i = 0; i < RPL; ++i) {
-        const bool vb = (cur >> i) & 1u;
-        const bool sb = (spike >> i) & 1u;
-        bc += vb ? T[i].wcos : 0.f;
-        bs += vb ? T[i].wsin : 0.f;
-        ec += sb ? T[i].cos : 0.f;
-        es += sb ? T[i].sin : 0.f;
-    }
-}
-
-// The agents [s0, s0 + n) of one replica, n <= kChainSeg.
+// The agents [s0, s0 + n) of one replica, n <= kChainSeg, one barrier per
+// agent k:
+//   * wave 0 takes agent k's sums (made by wave 1 during agent k - 1), moves
+//     agent k and decides the pairs (k + 1, k) and (k + 2, k) at k's new pose
+//     (one lane each);
+//   * wave 1 sums agent k + 1's field, which then lacks only the pair
+//     (k + 1, k), as the push kernel's wave 0 sums (one_wave_sums);
+//   * waves 2..7 decide agent k - 1's new pose in the fields of the
+//     segment's agents from k + 2 on (the cheap float32 cut-off test, the rest
+//     queued and decided 64 at a time).
+// When the pair (k + 1, k) draws, wave 1 sums agent k + 1's field again,
+// behind an extra barrier, before wave 0 uses it.  The decisions are exact
+// and the sums and moves are the push kernel's, so the two give the same
+// fields and poses bit for bit.
 template <int RPL>
 __global__ void __launch_bounds__(kChainThreads)
 vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
@@ -1456,93 +1443,73 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
     __shared__ uint32_t rows[kChainSeg][64];
     __shared__ float4 srec[kChainSeg];     // start poses (x, y, radius, theta)
     __shared__ float2 su[kChainSeg];       // start heading vectors
-    __shared__ double red[4 * kChainSumWaves];
+    __shared__ float4 sdyn[kChainSeg];     // speed, ALP0, BET0, V0
+    __shared__ double red[2][6];           // Sv, Sp, Bc, Bs, Ec, Es of agents k and k + 1
     __shared__ float4 newpose[2];
+    __shared__ int dirty[2];               // the pair (k, k - 1) drew after k's sums were taken
     __shared__ uint32_t queue[kChainWaves][kChainQueue];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
     const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
     const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
     for (int i = tid; i < n; i += kChainThreads) {
         const int g = C.a0 + s0 + i;
         srec[i] = make_float4(st.x[g], st.y[g], (float)st.radius[g], st.theta[g]);
         su[i] = ustart[s0 + i];
+        sdyn[i] = make_float4(st.v[g], st.alp0 ? st.alp0[g] : a0d, st.bet0 ? st.bet0[g] : b0d,
+                              st.v0 ? st.v0[g] : v0d);
     }
     for (int w = tid; w < n * 64; w += kChainThreads) {
         const int i = w >> 6, c = w & 63;
         rows[i][c] = c < C.W ? fields[(size_t)(s0 + i) * C.W + c] : 0u;
     }
-    // this thread's rays: those of the 16-wave kernel's threads tid and
-    // tid + 512, and their table entries
-    const int rs0 = tid * RPL, rs1 = (tid + kChainThreads) * RPL;
-    const int rn0 = rs0 >= C.R ? 0 : (C.R - rs0 < RPL ? C.R - rs0 : RPL);
-    const int rn1 = rs1 >= C.R ? 0 : (C.R - rs1 < RPL ? C.R - rs1 : RPL);
-    RayTab32 T0[RPL], T1[RPL];
+    if (tid < 2) dirty[tid] = 0;
+    // wave 1's lane l owns rays [l RPL, (l + 1) RPL) and their table entries
+    RayTab32 T[RPL];
+    if (wave == 1) {
 #pragma unroll
-    for (int i = 0; i < RPL; ++i) {
-        const int r0 = rs0 + i, r1 = rs1 + i;
-        if (r0 < C.R) T0[i] = tab32[rp->table_offset + r0];
-        else T0[i].wcos = T0[i].wsin = T0[i].cos = T0[i].sin = 0.f;
-        if (r1 < C.R) T1[i] = tab32[rp->table_offset + r1];
-        else T1[i].wcos = T1[i].wsin = T1[i].cos = T1[i].sin = 0.f;
+        for (int i = 0; i < RPL; ++i) {
+            const int r = lane * RPL + i;
+            if (r < C.R) T[i] = tab32[rp->table_offset + r];
+            else T[i].wcos = T[i].wsin = T[i].cos = T[i].sin = 0.f;
+        }
     }
-    const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
-    // wave 0: agent k's speed and parameters, loaded one agent ahead
-    float vel_k = 0.f, alp0_k = a0d, bet0_k = b0d, v0_k = v0d;
-    if (wave == 0) {
-        const int g = C.a0 + s0;
-        vel_k = st.v[g];
-        if (st.alp0) alp0_k = st.alp0[g];
-        if (st.bet0) bet0_k = st.bet0[g];
-        if (st.v0) v0_k = st.v0[g];
-    }
+    // wave 1: the sums of local agent kk's field into red[buf]
+    auto sums = [&](int kk, int buf) {
+        const float4 d = sdyn[kk];
+        double Sv, Sp, Bc, Bs, Ec, Es;
+        one_wave_sums<RPL>(rows[kk], T, lane, C, want_diag, d.y, d.z, Sv, Sp, Bc, Bs, Ec, Es);
+        if (lane == 0) {
+            double* r = red[buf];
+            r[0] = Sv; r[1] = Sp; r[2] = Bc; r[3] = Bs; r[4] = Ec; r[5] = Es;
+        }
+    };
     unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
     uint32_t* wq = queue[wave];
     __syncthreads();
-    if (wave == 0) __builtin_amdgcn_s_setprio(3);   // the chain's wave issues first
+    if (wave == 1) sums(0, 0);   // agent s0's field is complete
+    __syncthreads();
+    if (wave == 0) __builtin_amdgcn_s_setprio(3);   // the chain's wave issues first,
+    if (wave == 1) __builtin_amdgcn_s_setprio(2);   // then the summing wave
     for (int kk = 0; kk < n; ++kk) {
         const int k = s0 + kk;
-        // ---- A: agent k's sums (every thread); agent k + 1's speed (wave 0)
-        float vel_n = 0.f, alp0_n = a0d, bet0_n = b0d, v0_n = v0d;
-        if (wave == 0 && kk + 1 < n) {
-            const int g = C.a0 + k + 1;
-            vel_n = st.v[g];
-            if (st.alp0) alp0_n = st.alp0[g];
-            if (st.bet0) bet0_n = st.bet0[g];
-            if (st.v0) v0_n = st.v0[g];
+        const int buf = kk & 1;
+        if (dirty[buf]) {   // uniform: agent k's field changed after its sums
+            if (wave == 1) sums(kk, buf);
+            chain_barrier();
         }
-        const uint32_t* Fk = rows[kk];
-        {
-            const bool case1 = (Fk[0] & 1u) == 1u && field_bit(Fk, C.R - 1) == 0u;
-            float bc, bs, ec, es;
-            chain_ray_sums<RPL>(Fk, T0, rs0, rn0, case1, C, bc, bs, ec, es);
-            const double Bc0 = wave_total(bc), Bs0 = wave_total(bs), Ec0 = wave_total(ec), Es0 = wave_total(es);
-            chain_ray_sums<RPL>(Fk, T1, rs1, rn1, case1, C, bc, bs, ec, es);
-            const double Bc1 = wave_total(bc), Bs1 = wave_total(bs), Ec1 = wave_total(ec), Es1 = wave_total(es);
-            if (lane == 0) {
-                double* r0 = red + 4 * wave;
-                double* r1 = red + 4 * (wave + kChainWaves);
-                r0[0] = Bc0; r0[1] = Bs0; r0[2] = Ec0; r0[3] = Es0;
-                r1[0] = Bc1; r1[1] = Bs1; r1[2] = Ec1; r1[3] = Es1;
-            }
-        }
-        if (bits_out && tid < C.W) bits_out[C.field_offset + (int64_t)k * C.W + tid] = Fk[tid];
-        chain_barrier();
-        // ---- B
         if (wave == 0) {
             const int gk = C.a0 + k;
-            double Bc = 0.0, Bs = 0.0, Ec = 0.0, Es = 0.0;
-            for (int w = 0; w < kChainSumWaves; ++w) {
-                Bc += red[4 * w]; Bs += red[4 * w + 1]; Ec += red[4 * w + 2]; Es += red[4 * w + 3];
-            }
+            const double* r = red[buf];
+            const double Sv = r[0], Sp = r[1], Bc = r[2], Bs = r[3], Ec = r[4], Es = r[5];
             const float4 pk = srec[kk];
+            const float4 dk = sdyn[kk];
             const float fx = pk.x, fy = pk.y, fr = pk.z, fth = pk.w;
-            const float alp0 = alp0_k, bet0 = bet0_k, v0 = v0_k;
-            const double Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
-            const double Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
+            const float alp0 = dk.y, bet0 = dk.z, v0 = dk.w;
             float vel, o, dv, dpsi;
-            steer(C, fth, vel_k, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
+            steer(C, fth, dk.x, v0, (float)Sv, (float)Sp, &o, &vel, &dv, &dpsi);
             float so, co;
             sincos_f32(o, &so, &co);
             float nx, ny;
@@ -1562,46 +1529,61 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
                     dd[4] = C.phi_ok ? (float)((double)bet0 * -Bs) : 0.f;
                     dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
                 }
-                if (kk + 1 < n) {
-                    const float4 pn = srec[kk + 1];
-                    if (push_culled(pn, moved, C, cull_c))
-                        push_culled_check(pn, moved, rp, count, verify, n_pairs, n_disagree);
-                    else
-                        push_pair(rows[kk + 1], su[kk + 1], pn, moved, rp, C, count, verify, n_pairs, n_recheck,
-                                  n_disagree);
-                }
             }
-            vel_k = vel_n; alp0_k = alp0_n; bet0_k = bet0_n; v0_k = v0_n;
-        } else if (kk >= 1) {
-            // agent k - 1's new pose into the fields of the segment's agents
-            // after k: the cut-off test 64 at a time, the rest queued in a
-            // ring of 128 and decided 64 at a time
-            const float4 o = newpose[(kk - 1) & 1];
-            int head = 0, tail = 0;   // wave-uniform ring positions
-            for (int mb = kk + 1 + (wave - 1) * 64; mb < n; mb += kChainHelpers * 64) {
-                const int m = mb + lane;
-                bool keep = false;
-                if (m < n) {
-                    const float4 f = srec[m];
-                    if (!(o.x == f.x && o.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
-                        keep = !push_culled(f, o, C, cull_c);
-                        if (!keep) push_culled_check(f, o, rp, count, verify, n_pairs, n_disagree);
+            // lane d (1, 2): the pair (k + d, k)
+            bool drew = false;
+            if (lane >= 1 && lane <= 2 && kk + lane < n) {
+                const int m = kk + lane;
+                const float4 f = srec[m];
+                if (push_culled(f, moved, C, cull_c)) {
+                    push_culled_check(f, moved, rp, count, verify, n_pairs, n_disagree);
+                } else {
+                    PairF32 pr;
+                    if (decide_pair(pr, su[m], f, moved, rp, C, count, verify, n_pairs, n_recheck, n_disagree)) {
+                        raster(rows[m], pr.start, pr.end, C.R);
+                        drew = true;
                     }
                 }
-                const uint64_t mk = __ballot(keep);
-                if (keep) wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kChainQueue - 1)] = (uint32_t)m;
-                tail += (int)__popcll(mk);
-                wave_lds_sync();
-                if (tail - head >= 64) {
+            }
+            const uint64_t dm = __ballot(drew && lane == 1);
+            if (lane == 0) dirty[buf ^ 1] = dm != 0 ? 1 : 0;
+        } else if (wave == 1) {
+            if (kk + 1 < n) sums(kk + 1, buf ^ 1);   // lacks only the pair (k + 1, k)
+        } else {
+            if (bits_out && wave == 2 && lane < C.W) bits_out[C.field_offset + (int64_t)k * C.W + lane] = rows[kk][lane];
+            if (kk >= 1) {
+                // agent k - 1's new pose into the fields of the segment's
+                // agents from k + 2 on: the cut-off test 64 at a time, the rest
+                // queued in a ring of 128 and decided 64 at a time
+                const float4 o = newpose[(kk - 1) & 1];
+                int head = 0, tail = 0;   // wave-uniform ring positions
+                for (int mb = kk + 2 + (wave - 2) * 64; mb < n; mb += (kChainWaves - 2) * 64) {
+                    const int m = mb + lane;
+                    bool keep = false;
+                    if (m < n) {
+                        const float4 f = srec[m];
+                        if (!(o.x == f.x && o.y == f.y)) {   // coincident pairs are skipped (vf_supcalc.py:57)
+                            keep = !push_culled(f, o, C, cull_c);
+                            if (!keep) push_culled_check(f, o, rp, count, verify, n_pairs, n_disagree);
+                        }
+                    }
+                    const uint64_t mk = __ballot(keep);
+                    if (keep)
+                        wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kChainQueue - 1)] = (uint32_t)m;
+                    tail += (int)__popcll(mk);
+                    wave_lds_sync();
+                    if (tail - head >= 64) {
+                        const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
+                        push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck,
+                                  n_disagree);
+                        head += 64;
+                        wave_lds_sync();
+                    }
+                }
+                if (lane < tail - head) {
                     const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
                     push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
-                    head += 64;
-                    wave_lds_sync();
                 }
-            }
-            if (lane < tail - head) {
-                const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
-                push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
             }
         }
         chain_barrier();
@@ -1724,16 +1706,19 @@ int launch_chain_start(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t fla
     return check_hip(hipGetLastError(), "vf_step_f32_wide_kernel (start pairs) launch");
 }
 
-// Whether the chain step runs the batch: every replica above kWideThreshold
-// agents (a single large flock), the reference order, R up to 2048.
-bool chain_fits(const abm_vf_batch* b) {
-    return b->chain_fields != nullptr && b->chain_u != nullptr && b->n_replicas > 0 &&
-           b->min_agents > kWideThreshold && b->max_rays <= 2 * kBigThreads;
+// Whether the chain step can run the batch (its buffers, R up to 2048), and
+// whether it does by default: every replica above kWideThreshold agents (a
+// single large flock).
+bool chain_ready(const abm_vf_batch* b) {
+    return b->chain_fields != nullptr && b->chain_u != nullptr && b->grid_sorted != nullptr &&
+           b->grid_start != nullptr && b->grid_param != nullptr && b->n_replicas > 0 && b->max_rays <= 2048;
 }
+bool chain_fits(const abm_vf_batch* b) { return chain_ready(b) && b->min_agents > kWideThreshold; }
 
 int launch_chain(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag, uint32_t flags,
                  hipStream_t s) {
     const bool counting = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0;
+    const int rpl = push_rpl(b->max_rays);
     for (int rep = 0; rep < b->n_replicas; ++rep) {
         const VFRep& h = b->reps_host[rep];
         const int N = h.n_agents;
@@ -1750,14 +1735,14 @@ int launch_chain(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, fl
         if (rc) return rc;
         for (int s0 = 0; s0 < N; s0 += kChainSeg) {
             const int n = std::min(kChainSeg, N - s0);
-            if (h.n_rays <= kBigThreads)   // rays per 16-wave-kernel thread, as that kernel
-                hipLaunchKernelGGL(vf_step_f32_segment_kernel<1>, dim3(1), dim3(kChainThreads), 0, s,
-                                   b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,
-                                   (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
-            else
-                hipLaunchKernelGGL(vf_step_f32_segment_kernel<2>, dim3(1), dim3(kChainThreads), 0, s,
-                                   b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,
-                                   (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
+            // rays per lane of the summing wave: the push kernel's rule for the batch
+#define ABM_SEGMENT(RPLV)                                                                                   \
+    if (rpl == RPLV)                                                                                        \
+        hipLaunchKernelGGL(vf_step_f32_segment_kernel<RPLV>, dim3(1), dim3(kChainThreads), 0, s,           \
+                           b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,             \
+                           (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
+            ABM_SEGMENT(4) ABM_SEGMENT(8) ABM_SEGMENT(16) ABM_SEGMENT(32)
+#undef ABM_SEGMENT
             if (int rc2 = check_hip(hipGetLastError(), "vf_step_f32_segment_kernel launch")) return rc2;
             const int s1 = s0 + n;
             if (s1 >= N) break;
@@ -1907,6 +1892,11 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
                          "grouped kernel only, which this batch does not take (one R up to %d rays, "
                          "replicas of at most %d agents)", kGroupMaxRays, kGroupMaxAgents);
     const int nw = b->waves_per_group;
+    if (nw == 64 && !sync_order) {   // the chain step for every replica
+        if (!chain_ready(b))
+            return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: the chain step has no buffers (abm_vf_set_launch(64))");
+        return launch_chain(b, st, bits, diag, flags, s);
+    }
     if (nw == 32)   // the push kernel for every replica
         return launch_push_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, true);
     if (nw == 16)   // the 16-wave kernel for every replica

@@ -227,9 +227,12 @@ class VFSwarm:
         """Per-replica fused kernel with this many waves per replica (1, 2, 4),
         the 16-wave kernel (one workgroup per replica) for every replica (16),
         the push kernel (one workgroup per replica, pairs off the id-order
-        chain) for every replica (32), or the automatic choice (0: replicas
-        above 128 agents on the push kernel where their fields fit in LDS,
-        else the 16-wave kernel, the rest on the per-replica kernel)."""
+        chain) for every replica (32), the chain step (segments of agents
+        moved in id order, pairs decided over the whole GPU) for every replica
+        in the reference order (64), or the automatic choice (0: single flocks
+        above 2048 agents on the chain step; otherwise replicas above 128
+        agents on the push kernel where their fields fit in LDS, else the
+        16-wave kernel, the rest on the per-replica kernel)."""
         L.check(self.batch.lib.abm_vf_set_launch(self.batch.handle, waves_per_group))
 
     def last_kernels(self):

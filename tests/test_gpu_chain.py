@@ -2,12 +2,14 @@
 (vf_sims.py:302 -> vf_agent.py:52-77).
 
 Each agent's pairs with the agents after it are decided at their start poses
-over the whole GPU first; one workgroup then moves the agents in id order,
-deciding each moved agent's new pose in the fields of the agents after it
-(through the grid of start poses) while the next agents move.  It sums the
-same rays per thread in the same order as the 16-wave kernel (every pose in
-LDS, each agent's pairs decided in its turn) and every decision is exact, so
-the two must agree bit for bit: fields, poses, speeds and the integrals."""
+over the whole GPU first; the agents are then moved in id order in segments of
+256 (one workgroup), and each segment's new poses are decided in the fields of
+every later agent over the whole GPU (through the grid of start poses).  Every
+decision is exact and the segments sum and move as the push kernel does, so
+with the chain step forced on a flock the push kernel also holds (up to 2048
+agents) the two agree bit for bit: fields, poses, speeds, the integrals.
+Against the 16-wave kernel, which adds the same ray values in another order,
+the poses agree to float32 rounding."""
 import numpy as np
 import pytest
 
@@ -36,45 +38,80 @@ def flock(N, R, side, seed, boundary="infinite", seam=False, radius=2, params=Fa
     return cfg, (f32(x), f32(y), f32(th), f32(v)), radius, extra
 
 
-CASES = {
-    # C5 itself: grid path, two rays per thread
-    "c5": dict(N=8192, R=2048, side=4000.0, seed=11),
-    # agents crowded on the seam of a 3 x 3-cell torus, one ray per thread
-    "seam": dict(N=2304, R=1024, side=1500.0, seed=5, seam=True, radius=3),
-    # walls: no grid, every agent after k tested
-    "walls": dict(N=3000, R=512, side=2400.0, seed=8, boundary="walls", radius=5),
+PUSH_CASES = {
+    # (sizes the push kernel holds: every field in one CU's LDS)
+    # two segments on a grid (torus, three cells a side)
+    "grid": dict(N=450, R=2048, side=2000.0, seed=11),
+    # agents crowded on the seam of a 3 x 3-cell torus, four segments
+    "seam": dict(N=800, R=1024, side=1500.0, seed=5, seam=True, radius=3),
+    # walls: no grid, every later agent tested
+    "walls": dict(N=1200, R=512, side=1400.0, seed=8, boundary="walls", radius=5),
     # a torus too small for three cells a side: no grid
-    "small_torus": dict(N=2500, R=2048, side=1200.0, seed=9),
+    "small_torus": dict(N=400, R=2048, side=1000.0, seed=9),
     # radii 1..6 (the grid's reach follows the largest) and per-agent parameters
+    "mixed": dict(N=600, R=1536, side=4500.0, seed=12, radius="mixed", params=True),
+    # one partial segment
+    "short": dict(N=300, R=1024, side=600.0, seed=4, radius=4),
+}
+
+
+@pytest.mark.parametrize("diag", [False, True])
+@pytest.mark.parametrize("case", list(PUSH_CASES))
+def test_chain_matches_push_kernel(gpu, case, diag):
+    from abm_amd.vf import VFSwarm
+    cfg, st, radius, extra = flock(**PUSH_CASES[case])
+    a = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    b = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
+    a.set_launch(64)
+    b.set_launch(32)
+    for step in range(3):
+        print(f"  chain {case} diag={diag} step {step}", flush=True)
+        a.step(order="sequential", fields=True, diag=diag)
+        b.step(order="sequential", fields=True, diag=diag)
+        assert a.last_kernels() == {"chain"} and b.last_kernels() == {"push"}
+        assert np.array_equal(a.fields_numpy(0), b.fields_numpy(0)), step
+        for p, q in zip(a.state_numpy(), b.state_numpy()):
+            assert np.array_equal(p, q), step
+        if diag:
+            assert np.array_equal(a.diag_numpy(), b.diag_numpy()), step
+
+
+BIG_CASES = {
+    "c5": dict(N=8192, R=2048, side=4000.0, seed=11),
+    "walls": dict(N=3000, R=512, side=2400.0, seed=8, boundary="walls", radius=5),
     "mixed": dict(N=4096, R=1536, side=3000.0, seed=12, radius="mixed", params=True),
 }
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_chain_matches_16_wave_kernel(gpu, case):
+@pytest.mark.parametrize("case", list(BIG_CASES))
+def test_chain_agrees_with_16_wave_kernel(gpu, case):
+    # one step from the same state: the fields equal but for the few agents
+    # whose bins the slightly different poses of earlier agents tip (one ray
+    # changes an integral by about R^-1); the other agents' poses within
+    # float32 rounding of the sums' order (relative 1e-5)
     from abm_amd.vf import VFSwarm
-    cfg, st, radius, extra = flock(**CASES[case])
+    from test_gpu_vf_f32 import pose_err
+    cfg, st, radius, extra = flock(**BIG_CASES[case])
     a = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
     b = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
     b.set_launch(16)
-    for step in range(3):
-        print(f"  chain {case} step {step}", flush=True)
-        a.step(order="sequential", fields=True, diag=True)
-        b.step(order="sequential", fields=True, diag=True)
-        assert a.last_kernels() == {"chain"} and b.last_kernels() == {"big"}
-        assert np.array_equal(a.fields_numpy(0), b.fields_numpy(0)), step
-        for p, q in zip(a.state_numpy(), b.state_numpy()):
-            assert np.array_equal(p, q), step
-        assert np.array_equal(a.diag_numpy(), b.diag_numpy()), step
+    a.step(order="sequential", fields=True)
+    b.step(order="sequential", fields=True)
+    assert a.last_kernels() == {"chain"} and b.last_kernels() == {"big"}
+    same = np.all(a.fields_numpy(0) == b.fields_numpy(0), axis=1)
+    print(f"  {case}: {int((~same).sum())} of {cfg.n_agents} fields differ", flush=True)
+    assert (~same).sum() <= max(2, cfg.n_agents // 1000)
+    sa, sb = a.state_numpy(), b.state_numpy()
+    assert pose_err(tuple(q[same] for q in sa), tuple(q[same] for q in sb), cfg) <= 1e-5
 
 
-@pytest.mark.parametrize("case", ["c5", "seam", "walls"])
+@pytest.mark.parametrize("case", ["c5", "walls"])
 def test_chain_verify_mode(gpu, case):
     # counters and verify mode take the full scans: every pair is counted
     # once, every float32 decision and every dropped pair agrees with float64,
     # and the result equals the default path's
     from abm_amd.vf import VFSwarm
-    cfg, st, radius, extra = flock(**CASES[case])
+    cfg, st, radius, extra = flock(**BIG_CASES[case])
     a = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
     b = VFSwarm([cfg], *st, radius=radius, precision="f32", **extra)
     a.step(order="sequential", fields=True, verify=True)
@@ -89,14 +126,15 @@ def test_chain_verify_mode(gpu, case):
 
 
 def test_chain_sampled_agents_match_oracle(gpu):
-    # N = 2500 > 2048 on a 2400 px torus (grid path): sampled agents'
-    # fields and poses against the oracle on the poses each agent saw
+    # N = 2500 > 2048 on a 2400 px torus (grid path): sampled agents' fields
+    # and poses against the oracle on the poses each agent saw, among them
+    # both sides of segment boundaries
     from abm_amd.vf import VFSwarm
     from test_gpu_vf_f32 import per_agent_check
     cfg, st, radius, extra = flock(N=2500, R=2048, side=2400.0, seed=3)
     sw = VFSwarm([cfg], *st, radius=radius, precision="f32")
     sw.step(order="sequential", fields=True, diag=True)
     assert sw.last_kernels() == {"chain"}
-    agents = sorted({0, 1, 2, 1249, 2497, 2498, 2499} |
+    agents = sorted({0, 1, 2, 255, 256, 257, 511, 512, 1249, 2303, 2304, 2497, 2498, 2499} |
                     set(np.random.RandomState(1).choice(2500, 12, replace=False).tolist()))
     per_agent_check(sw, 0, cfg, st, sw.state_numpy(), radius=2, agents=agents)
