@@ -1074,14 +1074,16 @@ __device__ __forceinline__ void one_wave_sums(const uint32_t* Fk, const RayTab32
                 spike = (cur ^ ((cur >> 1) | (next << (rn - 1)))) & m;
             }
         }
+        // acc + (bit ? t : 0) as fma(bit, t, acc): the same value (a 0 bit adds
+        // a zero to an accumulator that is never -0), without a lane mask
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
-            const bool vb = (cur >> i) & 1u;
-            const bool sb = (spike >> i) & 1u;
-            bc += vb ? T[i].wcos : 0.f;
-            bs += vb ? T[i].wsin : 0.f;
-            ec += sb ? T[i].cos : 0.f;
-            es += sb ? T[i].sin : 0.f;
+            const float vb = (float)((cur >> i) & 1u);
+            const float sb = (float)((spike >> i) & 1u);
+            bc = __builtin_fmaf(vb, T[i].wcos, bc);
+            bs = __builtin_fmaf(vb, T[i].wsin, bs);
+            ec = __builtin_fmaf(sb, T[i].cos, ec);
+            es = __builtin_fmaf(sb, T[i].sin, es);
         }
     }
     Bc = Bs = Ec = Es = 0.0;
