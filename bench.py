@@ -232,11 +232,18 @@ def other_workload(args):
             step = staticmethod(lambda: sw.step(order="synchronous"))
         ms = time_steps(Sync, args.steps, args.warmup, not args.no_graph)[0]
         iqr = getattr(time_steps, "last_iqr", None)
-        # the reference's order: one workgroup, agents in id order
+        # the reference's order: the chain step (segments of 256 agents moved
+        # in id order by one workgroup, the pairs decided over the GPU)
         sq = VFSwarm([cfg], x, y, th, 0.5, radius=2, precision="f32")
         k_seq = max(5, min(args.steps // 5, 40))
         ms_seq = time_steps(sq, k_seq, min(args.warmup, 5), not args.no_graph)[0]
         iqr_seq = getattr(time_steps, "last_iqr", None)
+        kernels_seq = sorted(sq.last_kernels())
+        # the same order on the 16-wave kernel (one workgroup, every pose in
+        # LDS, each agent's pairs decided in its turn), for comparison
+        sb = VFSwarm([cfg], x, y, th, 0.5, radius=2, precision="f32")
+        sb.set_launch(16)
+        ms_big = time_steps(sb, 5, 2, not args.no_graph)[0]
         # pairs a step must decide: those inside the exact visibility cut-off
         # r cot(2 pi / R) (beyond it floor(proj_size / 2) = 0 and nothing is
         # drawn, vf_supcalc.py:116-117), torus images, from the start state
@@ -255,14 +262,16 @@ def other_workload(args):
         note = ("lane-ops of the pairs inside the visibility cut-off (40 each) and of every agent's "
                 "rays (17 each) and update (20)")
         extra = {"cpu_baseline": cpu} if cpu is not None else {}
-        _line(args, f"one flock N={Nl}, R={Rl}, radius 2, 4000x4000 torus, reference order (one "
-                    f"workgroup, agents in id order)",
+        _line(args, f"one flock N={Nl}, R={Rl}, radius 2, 4000x4000 torus, reference order (chain "
+                    f"step: agents moved in id order, pairs decided over the GPU)",
               Nl / (ms_seq * 1e-3), ms_seq, "f32", {"replicas": 1, "agents_per_replica": Nl, "rays": Rl,
                                                      "order": "sequential"},
-              ms_per_step_iqr=iqr_seq,
+              ms_per_step_iqr=iqr_seq, kernels=kernels_seq,
+              sixteen_wave_kernel={"ms_per_step": ms_big, "speedup": ms_big / ms_seq},
               roofline={"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                         "frac": ach / VALU_PEAK_TOPS, "traffic": None,
-                        "note": note + "; one workgroup runs the id-order chain: latency-bound"},
+                        "note": note + "; the id-order chain runs on one workgroup (one segment at a "
+                                       "time): latency-bound"},
               synchronous_order={"agent_steps_per_s": Nl / (ms * 1e-3), "ms_per_step": ms,
                                  "ms_per_step_iqr": iqr, "kernel": "wide (grid cells, one wave per agent)",
                                  "roofline_frac": ach_sync / VALU_PEAK_TOPS,

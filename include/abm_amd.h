@@ -168,17 +168,23 @@ int abm_vf_f64_near_ties(abm_vf_batch* batch, int64_t* near_ties, void* stream);
  * 1, 2 or 4 replicas advanced in lockstep by one wave (for batches with one
  * n_rays <= 1024 and at most 256 agents per replica; replicas that differ in
  * more than N and the ALP0/BET0/V0 defaults never share a wave).  Two more
- * give one workgroup of 1024 threads to each replica: the 16-wave kernel
- * (every pose in LDS, the pairs of each agent decided in its turn), which
- * runs single flocks above 2048 agents in the reference order, and the push
- * kernel (every agent's field in LDS, the pairs decided off the id-order
- * chain as soon as their object has moved), which by default runs the large
- * size class of a batch (replicas above 128 agents) when their fields fit,
- * next to the per-replica kernel for the rest.  abm_vf_set_launch selects
- * the per-replica kernel with waves_per_group waves per replica (1, 2, 4),
- * the 16-wave kernel for every replica (16), the push kernel for every
- * replica (32; ABM_ERR_UNSUPPORTED at step time when the fields do not fit
- * in LDS), or the automatic choice (0). */
+ * give one workgroup to each replica: the 16-wave kernel (every pose in LDS,
+ * the pairs of each agent decided in its turn) and the push kernel (every
+ * agent's field in LDS, the pairs decided off the id-order chain as soon as
+ * their object has moved), which by default runs the large size class of a
+ * batch (replicas above 128 agents) when their fields fit, next to the
+ * per-replica kernel for the rest, and the 16-wave kernel when they do not.
+ * Batches of flocks above 2048 agents in the reference order take the chain
+ * step: each agent's pairs with later agents at their start poses over the
+ * whole GPU, then segments of 256 agents moved in id order by one workgroup,
+ * each segment's new poses decided in the later agents' fields over the
+ * whole GPU (the push kernel's sums and moves, so the same results).
+ * abm_vf_set_launch selects the per-replica kernel with waves_per_group waves
+ * per replica (1, 2, 4), the 16-wave kernel for every replica (16), the push
+ * kernel for every replica (32; ABM_ERR_UNSUPPORTED at step time when the
+ * fields do not fit in LDS), the chain step for every replica in the
+ * reference order (64; allocates its buffers when the batch has none: call it
+ * outside graph capture), or the automatic choice (0). */
 int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
 /* The kernels the batch's last step launched (ABM_KERNEL_* bits), for
  * callers that log or check the launch choice.  No counterpart in the
