@@ -996,7 +996,8 @@ int launch_big(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, floa
     if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: %d agents in one replica need %zu B of LDS in the reference "
-                         "(sequential) order (device max %d); use ABM_ORDER_SYNCHRONOUS", maxN,
+                         "(sequential) order on the 16-wave kernel (device max %d); the chain step (the default "
+                         "for single flocks, abm_vf_set_launch(64)) takes it", maxN,
                          lds, b->dev_max_lds);
     auto kern = vf_step_f32_big_kernel<RPL>;
     if (lds > 65536) {

@@ -138,3 +138,28 @@ def test_chain_sampled_agents_match_oracle(gpu):
     agents = sorted({0, 1, 2, 255, 256, 257, 511, 512, 1249, 2303, 2304, 2497, 2498, 2499} |
                     set(np.random.RandomState(1).choice(2500, 12, replace=False).tolist()))
     per_agent_check(sw, 0, cfg, st, sw.state_numpy(), radius=2, agents=agents)
+
+
+def test_chain_beyond_the_16_wave_kernels_lds(gpu):
+    # 12,000 agents (R = 1024, C5's density): more poses than the 16-wave
+    # kernel holds in LDS (about 8,900), which the chain step does not need
+    # (its fields live in global memory).  One step in verify mode (every
+    # pair counted once, none disagreeing with float64), then one with fields
+    # whose sampled agents match the oracle on the poses they saw.
+    from abm_amd import _lib as L
+    from abm_amd.vf import VFSwarm
+    from test_gpu_vf_f32 import per_agent_check
+    cfg, st, radius, extra = flock(N=12000, R=1024, side=4800.0, seed=21)
+    big = VFSwarm([cfg], *st, radius=radius, precision="f32")
+    big.set_launch(16)
+    with pytest.raises(L.AbmUnsupported, match="LDS"):
+        big.step(order="sequential")
+    sw = VFSwarm([cfg], *st, radius=radius, precision="f32")
+    sw.step(order="sequential", verify=True)
+    assert sw.last_kernels() == {"chain"}
+    pairs, _, disagree = sw.fallback_counts()
+    assert disagree == 0 and pairs == 12000 * 11999
+    before = sw.state_numpy()
+    sw.step(order="sequential", fields=True, diag=True)
+    agents = [0, 1, 255, 256, 6000, 11775, 11776, 11998, 11999]
+    per_agent_check(sw, 0, cfg, before, sw.state_numpy(), radius=2, agents=agents)
