@@ -313,8 +313,9 @@ def other_workload(args):
                                                 max_resc_quality=1.0, teleport_exploit=False)
                                     for _ in fg_cfgs], device_rng=True)
         world.create_resources()
-        vsw.autotune(trial_steps=2)
+        chosen, chosen_ms = vsw.autotune(trial_steps=3)
         ms_vf = time_steps(vsw, args.steps, args.warmup, not args.no_graph)[0]
+        vf_kernels = sorted(vsw.last_kernels())
         for _ in range(min(args.warmup, 3)):
             world.step()
         torch.cuda.synchronize()
@@ -332,7 +333,7 @@ def other_workload(args):
         ach = ops_vf / (ms_vf * 1e-3) / 1e12
         extra = {"roofline": {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                               "frac": ach / VALU_PEAK_TOPS, "traffic": None,
-                              "kernel": "flocking half (per-replica fused kernel), lane-ops of every "
+                              "kernel": "flocking half (the autotuned launch), lane-ops of every "
                                         "flocking agent-step over its graph-replay time"}}
         if cpu is not None:
             extra["cpu_baseline"] = cpu
@@ -343,7 +344,10 @@ def other_workload(args):
               {"replicas": 2000, "agents_flocking": n_vf, "agents_foraging": n_fg},
               flocking={"agent_steps_per_s": n_vf / (ms_vf * 1e-3), "ms_per_step": ms_vf,
                         "ms_per_step_iqr": getattr(time_steps, "last_iqr", None),
-                        "launch": vsw.chosen_launch if hasattr(vsw, "chosen_launch") else None},
+                        "launch": {"autotuned": chosen, "trial_ms": chosen_ms, "kernels": vf_kernels,
+                                   "note": "abm_vf_set_launch value: 32 the push kernel for every "
+                                           "replica, 0 the size classes, 1/2/4 the per-replica "
+                                           "kernel, -1/-2/-4 the grouped kernel"}},
               foraging={"agent_steps_per_s": n_fg / (ms_fg * 1e-3), "ms_per_step_wall": ms_fg},
               **extra)
 
