@@ -181,12 +181,12 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
 #pragma unroll
                 for (int i = 0; i < RPL; ++i) {
                     const RayTab32 T = S.tab[i * NT + tid];
-                    const bool vb = (cur >> i) & 1u;
-                    const bool sb = (spike >> i) & 1u;
-                    bc += vb ? T.wcos : 0.f;
-                    bs += vb ? T.wsin : 0.f;
-                    ec += sb ? T.cos : 0.f;
-                    es += sb ? T.sin : 0.f;
+                    const float vb = (float)((cur >> i) & 1u);   // acc + (bit ? t : 0) as fma(bit, t, acc)
+                    const float sb = (float)((spike >> i) & 1u);
+                    bc = __builtin_fmaf(vb, T.wcos, bc);
+                    bs = __builtin_fmaf(vb, T.wsin, bs);
+                    ec = __builtin_fmaf(sb, T.cos, ec);
+                    es = __builtin_fmaf(sb, T.sin, es);
                 }
             }
             const float4 dk = S.dyn[k];
@@ -553,12 +553,12 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
 #pragma unroll 8
         for (int i = 0; i < RPL; ++i) {
             const RayTab32 T = S.tab[i * 64 + lane];
-            const bool vb = (cur >> i) & 1u;
-            const bool sb = (spike >> i) & 1u;
-            bc += vb ? T.wcos : 0.f;
-            bs += vb ? T.wsin : 0.f;
-            ec += sb ? T.cos : 0.f;
-            es += sb ? T.sin : 0.f;
+            const float vb = (float)((cur >> i) & 1u);   // acc + (bit ? t : 0) as fma(bit, t, acc)
+            const float sb = (float)((spike >> i) & 1u);
+            bc = __builtin_fmaf(vb, T.wcos, bc);
+            bs = __builtin_fmaf(vb, T.wsin, bs);
+            ec = __builtin_fmaf(sb, T.cos, ec);
+            es = __builtin_fmaf(sb, T.sin, es);
         }
     }
     const float alp0 = st.alp0 ? st.alp0[g] : (float)rp->alp0;
@@ -922,12 +922,12 @@ vf_step_f32_big_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict
             }
 #pragma unroll
             for (int i = 0; i < RPL; ++i) {
-                const bool vb = (cur >> i) & 1u;
-                const bool sb = (spike >> i) & 1u;
-                bc += vb ? T[i].wcos : 0.f;
-                bs += vb ? T[i].wsin : 0.f;
-                ec += sb ? T[i].cos : 0.f;
-                es += sb ? T[i].sin : 0.f;
+                const float vb = (float)((cur >> i) & 1u);   // acc + (bit ? t : 0) as fma(bit, t, acc)
+                const float sb = (float)((spike >> i) & 1u);
+                bc = __builtin_fmaf(vb, T[i].wcos, bc);
+                bs = __builtin_fmaf(vb, T[i].wsin, bs);
+                ec = __builtin_fmaf(sb, T[i].cos, ec);
+                es = __builtin_fmaf(sb, T[i].sin, es);
             }
         }
         const int gk = C.a0 + k;
