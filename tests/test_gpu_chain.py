@@ -163,3 +163,45 @@ def test_chain_beyond_the_16_wave_kernels_lds(gpu):
     sw.step(order="sequential", fields=True, diag=True)
     agents = [0, 1, 255, 256, 6000, 11775, 11776, 11998, 11999]
     per_agent_check(sw, 0, cfg, before, sw.state_numpy(), radius=2, agents=agents)
+
+
+def test_chain_batch_of_replicas(gpu):
+    # the chain step runs a batch replica by replica, reusing its grid and
+    # field buffers: a batch of three forced onto it equals the push kernel,
+    # and two flocks above 2048 agents in one batch equal each flock alone
+    # (one R per batch: the summing lanes' ray spans follow the batch's
+    # largest R)
+    from abm_amd.vf import VFSwarm
+    parts = [flock(N=300, R=1024, side=700.0, seed=41, radius=4),
+             flock(N=450, R=1024, side=2000.0, seed=42),
+             flock(N=129, R=1024, side=500.0, seed=43, boundary="walls", radius=3)]
+    cfgs = [p[0] for p in parts]
+    cat = [np.concatenate([p[1][i] for p in parts]) for i in range(4)]
+    rad = np.concatenate([np.full(p[0].n_agents, p[2]) for p in parts])
+    a = VFSwarm(cfgs, *cat, radius=rad, precision="f32")
+    b = VFSwarm(cfgs, *cat, radius=rad, precision="f32")
+    a.set_launch(64)
+    b.set_launch(32)
+    for step in range(2):
+        a.step(order="sequential", fields=True)
+        b.step(order="sequential", fields=True)
+        assert a.last_kernels() == {"chain"} and b.last_kernels() == {"push"}
+        for r in range(3):
+            assert np.array_equal(a.fields_numpy(r), b.fields_numpy(r)), (step, r)
+        for p, q in zip(a.state_numpy(), b.state_numpy()):
+            assert np.array_equal(p, q), step
+
+    big = [flock(N=2100, R=2048, side=2200.0, seed=44), flock(N=2600, R=2048, side=2500.0, seed=45)]
+    cat = [np.concatenate([p[1][i] for p in big]) for i in range(4)]
+    both = VFSwarm([p[0] for p in big], *cat, radius=2, precision="f32")
+    alone = [VFSwarm([p[0]], *p[1], radius=2, precision="f32") for p in big]
+    both.step(order="sequential", fields=True)
+    for s in alone:
+        s.step(order="sequential", fields=True)
+    assert both.last_kernels() == {"chain"}
+    st = both.state_numpy()
+    for r, s in enumerate(alone):
+        assert np.array_equal(both.fields_numpy(r), s.fields_numpy(0)), r
+        sl = both.replica_slice(r)
+        for p, q in zip(st, s.state_numpy()):
+            assert np.array_equal(p[sl], q), r
