@@ -105,6 +105,18 @@ def time_steps(sw, steps, warmup, use_graph=True, per_graph=10):
     return float(np.mean(per_step)), float(np.median(per_step)), wall * 1e3 / (reps * per)
 
 
+def valu_instructions_per_agent_step():
+    """VALU wave instructions the fused kernel issues per agent-step on this
+    workload (rocprofv3 SQ_INSTS_VALU per launch / agent-steps), recorded next
+    to the profile summaries; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_vswrm_sweep_4096x50x512.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["valu_wave_instructions_per_agent_step"]["f32"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def traffic_bytes():
     """HBM bytes per launch of the fused kernel on this workload, as measured by
     rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a device copy of known size)
@@ -839,8 +851,16 @@ def main():
             "frac": achieved / VALU_PEAK_TOPS,
             "traffic": traffic_bytes(),
             "ops_per_launch": ops,
+            "note": "achieved counts the op model's lane-ops (40 per pair, 17 per ray, 20 per "
+                    "agent: 10,684 per agent-step here), not what the bit-packed kernel issues; "
+                    "issue_frac is the issued VALU lane slots (profiled VALU wave instructions "
+                    "per agent-step x 64) over the same peak",
         },
     }
+    vpa = valu_instructions_per_agent_step()
+    if vpa is not None:
+        out["roofline"]["valu_wave_instructions_per_agent_step"] = vpa
+        out["roofline"]["issue_frac"] = vpa * 64 * agents / (ms32_mean * 1e-3) / 1e12 / VALU_PEAK_TOPS
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
