@@ -335,6 +335,7 @@ int abm_vf_batch_create(const abm_vf_replica* reps, int32_t n, abm_vf_batch** ou
     for (int b = 0; b < n; ++b) {
         if (host[b].n_agents > kBigClass) {
             ++bt->n_big_class;
+            bt->big_sum_agents += host[b].n_agents;
             bt->big_max_agents = std::max(bt->big_max_agents, host[b].n_agents);
             bt->big_max_rays = std::max(bt->big_max_rays, host[b].n_rays);
         } else {

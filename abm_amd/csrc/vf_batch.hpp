@@ -58,10 +58,10 @@ struct GroupPlan {
     int32_t tab_entries = 0;     // float2 entries per table
 };
 
-// Replicas above this many agents form the large size class of a ragged
-// batch: the 16-wave (one workgroup per replica) kernel updates them, the
-// per-replica kernel the rest.
-constexpr int kBigClass = 128;
+// Replicas above this many agents form the large size class of a batch:
+// the push kernel (or, where their fields do not fit in LDS, the 16-wave
+// kernel) updates them, the per-replica kernel the rest.
+constexpr int kBigClass = 64;
 
 // The push kernel (one workgroup of 8 waves per replica): rays per lane of
 // its summing wave for a launch whose largest R is r (the one-wave
@@ -117,6 +117,7 @@ struct abm_vf_batch {
     size_t push_lds_big = 0;        // ... of the largest in the large size class
     int32_t small_max_agents = 0, small_max_rays = 0;   // of the other replicas
     int32_t big_max_agents = 0, big_max_rays = 0;       // of the large class
+    int64_t big_sum_agents = 0;     // agents in the large class
     float4* wide_out = nullptr;     // new poses of the wide (large-flock) step
     float4* grid_sorted = nullptr;  // wide step: the replica's agents sorted by grid cell
     int32_t* grid_start = nullptr;  // wide step: first sorted agent of each cell (+ end)

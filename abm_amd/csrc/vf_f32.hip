@@ -1906,8 +1906,14 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
                               b->max_rays);
     const int nb = b->n_big_class;
-    const bool push = nb > 0 && push_fits(b, false);
-    if (nw == 0 && nb > 0 && (push || nb < b->n_replicas)) {
+    const bool push_ok = nb > 0 && push_fits(b, false);
+    // The push kernel holds one replica per CU (eight waves of up to 256
+    // VGPRs), so it shortens the step only while the large class's agents,
+    // spread over the CUs, do not outlast the chain of its largest replica;
+    // beyond that (many replicas of similar size) the per-replica kernel's
+    // occupancy wins.
+    const bool push = push_ok && b->big_sum_agents <= (int64_t)b->big_max_agents * b->n_cu;
+    if (nw == 0 && nb > 0 && (push || (!push_ok && nb < b->n_replicas))) {
         // size classes: the small replicas on the per-replica kernel, then
         // the large ones, whose id-order chains are the step's critical
         // path, on the push kernel (pairs off the chain) where their fields

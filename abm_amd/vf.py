@@ -230,9 +230,11 @@ class VFSwarm:
         chain) for every replica (32), the chain step (segments of agents
         moved in id order, pairs decided over the whole GPU) for every replica
         in the reference order (64), or the automatic choice (0: single flocks
-        above 2048 agents on the chain step; otherwise replicas above 128
-        agents on the push kernel where their fields fit in LDS, else the
-        16-wave kernel, the rest on the per-replica kernel)."""
+        above 2048 agents on the chain step; otherwise replicas above 64
+        agents on the push kernel where their fields fit in LDS and the
+        class's agents over the CUs do not outlast its largest replica's
+        chain, the 16-wave kernel where the fields do not fit, the rest on
+        the per-replica kernel)."""
         L.check(self.batch.lib.abm_vf_set_launch(self.batch.handle, waves_per_group))
 
     def last_kernels(self):

@@ -172,8 +172,10 @@ int abm_vf_f64_near_ties(abm_vf_batch* batch, int64_t* near_ties, void* stream);
  * the pairs of each agent decided in its turn) and the push kernel (every
  * agent's field in LDS, the pairs decided off the id-order chain as soon as
  * their object has moved), which by default runs the large size class of a
- * batch (replicas above 128 agents) when their fields fit, next to the
- * per-replica kernel for the rest, and the 16-wave kernel when they do not.
+ * batch (replicas above 64 agents) when their fields fit and the class's
+ * agents, spread over the CUs, are no more than its largest replica's chain
+ * (one push workgroup per CU), next to the per-replica kernel for the rest;
+ * the 16-wave kernel takes the class when its fields do not fit.
  * Batches of flocks above 2048 agents in the reference order take the chain
  * step: each agent's pairs with later agents at their start poses over the
  * whole GPU, then segments of 256 agents moved in id order by one workgroup,

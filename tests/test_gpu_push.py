@@ -76,7 +76,7 @@ def test_push_kernel_agents_match_oracle(gpu):
     assert sw.last_kernels() == {"per_replica", "push"}
     after = sw.state_numpy()
     for b, cfg in enumerate(cfgs):
-        if cfg.n_agents <= 128:
+        if cfg.n_agents <= 64:
             continue
         n = cfg.n_agents
         sl = sw.replica_slice(b)

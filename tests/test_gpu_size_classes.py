@@ -1,5 +1,5 @@
 """Launch shapes of ragged VSWRM batches: the size classes (replicas above
-128 agents on the 16-wave kernel, the rest on the per-replica kernel), the
+64 agents on the push kernel, the rest on the per-replica kernel), the
 16-wave kernel for every replica and the per-replica kernel alone.  Each
 decides every pair as float64 does (verify mode, three steps, no
 disagreement), and each agent of a sample matches the oracle on the poses
