@@ -1895,7 +1895,13 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
                          "grouped kernel only, which this batch does not take (one R up to %d rays, "
                          "replicas of at most %d agents)", kGroupMaxRays, kGroupMaxAgents);
     const int nw = b->waves_per_group;
-    if (nw == 64 && !sync_order) {   // the chain step for every replica
+    // the 16-wave and push kernels and the chain step move agents in the
+    // reference order only; the synchronous order takes the per-replica kernel
+    if (sync_order && (nw == 16 || nw == 32 || nw == 64))
+        return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32: the %s runs the reference order only "
+                         "(ABM_ORDER_SYNCHRONOUS takes the per-replica kernel, abm_vf_set_launch(0))",
+                         nw == 16 ? "16-wave kernel" : (nw == 32 ? "push kernel" : "chain step"));
+    if (nw == 64) {   // the chain step for every replica
         if (!chain_ready(b))
             return set_error(ABM_ERR_INVALID, "abm_vf_step_f32: the chain step has no buffers (abm_vf_set_launch(64))");
         return launch_chain(b, st, bits, diag, flags, s);
@@ -1906,14 +1912,14 @@ int launch_vf_step_f32(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
         return launch_big_any(b, st, bits, diag, flags, s, b->order_dev, b->n_replicas, b->max_agents,
                               b->max_rays);
     const int nb = b->n_big_class;
-    const bool push_ok = nb > 0 && push_fits(b, false);
+    const bool push_ok = !sync_order && nb > 0 && push_fits(b, false);
     // The push kernel holds one replica per CU (eight waves of up to 256
     // VGPRs), so it shortens the step only while the large class's agents,
     // spread over the CUs, do not outlast the chain of its largest replica;
     // beyond that (many replicas of similar size) the per-replica kernel's
     // occupancy wins.
     const bool push = push_ok && b->big_sum_agents <= (int64_t)b->big_max_agents * b->n_cu;
-    if (nw == 0 && nb > 0 && (push || (!push_ok && nb < b->n_replicas))) {
+    if (nw == 0 && nb > 0 && !sync_order && (push || (!push_ok && nb < b->n_replicas))) {
         // size classes: the small replicas on the per-replica kernel, then
         // the large ones, whose id-order chains are the step's critical
         // path, on the push kernel (pairs off the chain) where their fields

@@ -89,7 +89,10 @@ __device__ inline void raster_pred(uint32_t* F, int start, int end, int R, bool 
         if (long0) or_range(F, p0, p0 + n0);
 }
 
-template <int G, int RPL, bool KH>
+// PLAIN: a step with no fields, diagnostics or counters out (the flags and
+// pointers that ask for them are ignored), so their tests and addresses leave
+// the hot loop.
+template <int G, int RPL, bool KH, bool PLAIN>
 __global__ void __launch_bounds__(64 * (G == 1 ? 8 : 16 / G))
 __attribute__((amdgpu_waves_per_eu(G == 1 ? 4 : 1)))
 vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ groups, int n_groups,
@@ -105,9 +108,9 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
     const int nwg = blockDim.x >> 6;
     const int gl = lane / L, li = lane % L;
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
 
     float2* tabB = reinterpret_cast<float2*>(smem_d);
     float2* tabS = tabB + TAB;
@@ -172,130 +175,136 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
         float sth, cth;
         sincos_f32(fp.w, &sth, &cth);
 
+        // the focal agents in four priority stages (no per-agent test)
         const ProgressStages stages = progress_stages(nmax);
-        for (int k = 0; k < nmax; ++k) {
-            progress_priority(k, stages);
-            const bool kact = k < myN;
-            const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
-            const float u1x = cth, u1y = -sth;
+        int k = 0;
+        for (int stage = 0; stage < 4; ++stage) {
+            const int kend = stage_priority(stage, stages, nmax);
+            for (; k < kend; ++k) {
+                const bool kact = k < myN;
+                const float fx = fp.x, fy = fp.y, fr = fp.z, fth = fp.w;
+                const float u1x = cth, u1y = -sth;
 
-            // ------------------------------------------------ pairs
-            // every lane evaluates a pair (inactive lanes a valid dummy) and
-            // only the raster and the counters are predicated
-            int nre = 0;   // wave-uniform
-            for (int t = 0; t < passes; ++t) {
-                const int j = li + t * L;
-                const bool act = kact && j < myN && j != k;
-                const float4 o = objg[act ? j : 0];
-                PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
-                const bool live = act && !(o.x == fx && o.y == fy);     // coincident (:57)
-                if (count) n_pairs += __popcll(__ballot(live));         // wave total
-                // pairs outside the float32 decision (rare), every pair in verify mode
-                if (__ballot(live && !p.ok) || verify) {
-                    if (live && p.tie && tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, reps + rid, C))
-                        if (count) ++n_recheck;   // a float64 decision too
-                    const bool flag = live && (!p.ok || verify);
-                    const uint64_t fm = __ballot(flag);
-                    if (flag) {
-                        const int slot = nre + (int)__popcll(fm & ((1ull << lane) - 1ull));
-                        S.rlist[slot] = entry_code(j, gl, p.ok);
-                        S.rfast[slot] = fast_code(p.start, p.end, p.drawn, C.R);
+                // ------------------------------------------------ pairs
+                // every lane evaluates a pair (inactive lanes a valid dummy) and
+                // only the raster and the counters are predicated
+                int nre = 0;   // wave-uniform
+                for (int t = 0; t < passes; ++t) {
+                    const int j = li + t * L;
+                    const bool act = kact && j < myN && j != k;
+                    const float4 o = objg[act ? j : 0];
+                    PairF32 p = fast_pair(u1x, u1y, fx, fy, fr, o.x, o.y, o.z, C);
+                    const bool live = act && !(o.x == fx && o.y == fy);     // coincident (:57)
+                    if (count) n_pairs += __popcll(__ballot(live));         // wave total
+                    // pairs outside the float32 decision (rare), every pair in verify mode
+                    if (__ballot(live && !p.ok) || verify) {
+                        if (live && p.tie && tie_in_place(p, fx, fy, fth, fr, o.x, o.y, o.z, reps + rid, C))
+                            if (count) ++n_recheck;   // a float64 decision too
+                        const bool flag = live && (!p.ok || verify);
+                        const uint64_t fm = __ballot(flag);
+                        if (flag) {
+                            const int slot = nre + (int)__popcll(fm & ((1ull << lane) - 1ull));
+                            S.rlist[slot] = entry_code(j, gl, p.ok);
+                            S.rfast[slot] = fast_code(p.start, p.end, p.drawn, C.R);
+                        }
+                        nre += (int)__popcll(fm);
                     }
-                    nre += (int)__popcll(fm);
-                }
-                raster_pred(Fg, p.start, p.end, C.R, live && p.ok && p.drawn);
-            }
-            wave_sync();
-            if (nre > 0) {
-                for (int i = lane; i < nre; i += 64) {
-                    const int e = S.rlist[i];
-                    const int fc = S.rfast[i];
-                    const int j = e & 0xfffff;
-                    const int g = (e >> 20) & 0xff;
-                    const float4 f = S.pose[g * maxN + k];
-                    const float4 o = S.obj[g * maxN + j];
-                    const VFRep* rpg = reps + grp[g];
-                    int dis;
-                    const int r = resolve_queued(e, fc, f.x, f.y, f.w, f.z, o.x, o.y, o.z, rpg, verify, &dis);
-                    n_disagree += dis;
-                    if (e & kEntryForced) {
-                        if (count) ++n_recheck;
-                        if (r >= 0) raster(S.F + g * (W + 1), (r & 0x7fff) - 2 * C.R, ((r >> 15) & 0x7fff) - 2 * C.R, C.R);
-                    }
+                    raster_pred(Fg, p.start, p.end, C.R, live && p.ok && p.drawn);
                 }
                 wave_sync();
-            }
-
-            // ------------------------------------------------ sums
-            const int kn = k + 1 < nmax ? k + 1 : k;      // next focal agent
-            const float4 fpn = poseg[kn];
-            const float4 fdn = dyng[kn];
-            uint64_t v, sp;
-            span_bits<G, RPL>(Fg, li, C.R, &v, &sp);
-            if (bits_out && kact) {
-                uint32_t* dst = bits_out + fo + (int64_t)k * C.W;
-                for (int w = li; w < C.W; w += L) dst[w] = Fg[w];
-            }
-            wave_sync();
-            if (W + 1 <= L) {   // wave-uniform
-                if (li < W + 1) Fg[li] = 0u;
-            } else {
-                for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
-            }
-            float bc, bs, ec, es;
-            float Sv, Sp, Bc, Bs, Ec, Es;   // group totals in diag mode
-            if (KH) {
-                // Kahan-compensated ray sums over the lane's nibble entries,
-                // then the group's sums of the compensated values in float64
-                float ks[4], kc[4];
-                span_sums_kahan<RPL, L>(tb, ts, v, sp, ks, kc);
-                const double dBc = group_sum_f64<G>((double)ks[0] - (double)kc[0]);
-                const double dBs = group_sum_f64<G>((double)ks[1] - (double)kc[1]);
-                const double dEc = group_sum_f64<G>((double)ks[2] - (double)kc[2]);
-                const double dEs = group_sum_f64<G>((double)ks[3] - (double)kc[3]);
-                Sv = (float)((double)fd.y * -dBc + (double)(fd.y * C.alp1) * dEc);
-                Sp = (float)((double)fd.z * -dBs + (double)(fd.z * C.bet1) * dEs);
-                Bc = (float)dBc; Bs = (float)dBs; Ec = (float)dEc; Es = (float)dEs;
-            } else {
-            span_sums<RPL, L>(tb, ts, v, sp, &bc, &bs, &ec, &es);
-            Bc = bc; Bs = bs; Ec = ec; Es = es;
-            if (!want_diag) {
-                Sv = fd.y * (-bc) + (fd.y * C.alp1) * ec;
-                Sp = fd.z * (-bs) + (fd.z * C.bet1) * es;
-                group_sum2<G>(Sv, Sp);
-            } else {
-                group_sum2<G>(Bc, Bs);
-                group_sum2<G>(Ec, Es);
-                Sv = fd.y * -Bc + (fd.y * C.alp1) * Ec;
-                Sp = fd.z * -Bs + (fd.z * C.bet1) * Es;
-            }
-            }
-
-            // ------------------------------------------------ motion
-            float vel, o, dv, dpsi;
-            steer(C, fth, fd.x, fd.w, Sv, Sp, &o, &vel, &dv, &dpsi);
-            const float ang = li == L - 2 ? fpn.w : o;
-            float s_l, c_l;
-            sincos_f32(ang, &s_l, &c_l);
-            float nx, ny;
-            euler_boundary(C, fx, fy, fr, vel, s_l, c_l, &o, &nx, &ny);
-            if (li == L - 1 && kact) {
-                poseg[k] = make_float4(nx, ny, fr, o);
-                dyng[k].x = vel;
-                if (want_diag) {
-                    float* dd = diag + 6 * (int64_t)(a0 + k);
-                    dd[0] = dv; dd[1] = dpsi;
-                    dd[2] = C.phi_ok ? fd.y * -Bc : 0.f;
-                    dd[3] = C.phi_ok ? (fd.y * C.alp1) * Ec : 0.f;
-                    dd[4] = C.phi_ok ? fd.z * -Bs : 0.f;
-                    dd[5] = C.phi_ok ? (fd.z * C.bet1) * Es : 0.f;
+                if (nre > 0) {
+                    for (int i = lane; i < nre; i += 64) {
+                        const int e = S.rlist[i];
+                        const int fc = S.rfast[i];
+                        const int j = e & 0xfffff;
+                        const int g = (e >> 20) & 0xff;
+                        const float4 f = S.pose[g * maxN + k];
+                        const float4 o = S.obj[g * maxN + j];
+                        const VFRep* rpg = reps + grp[g];
+                        int dis;
+                        const int r = resolve_queued(e, fc, f.x, f.y, f.w, f.z, o.x, o.y, o.z, rpg, verify, &dis);
+                        n_disagree += dis;
+                        if (e & kEntryForced) {
+                            if (count) ++n_recheck;
+                            if (r >= 0)
+                                raster(S.F + g * (W + 1), (r & 0x7fff) - 2 * C.R, ((r >> 15) & 0x7fff) - 2 * C.R,
+                                       C.R);
+                        }
+                    }
+                    wave_sync();
                 }
+
+                // ------------------------------------------------ sums
+                const int kn = k + 1 < nmax ? k + 1 : k;      // next focal agent
+                const float4 fpn = poseg[kn];
+                const float4 fdn = dyng[kn];
+                uint64_t v, sp;
+                span_bits<G, RPL>(Fg, li, C.R, &v, &sp);
+                if (!PLAIN && bits_out && kact) {
+                    uint32_t* dst = bits_out + fo + (int64_t)k * C.W;
+                    for (int w = li; w < C.W; w += L) dst[w] = Fg[w];
+                }
+                wave_sync();
+                if (W + 1 <= L) {   // wave-uniform
+                    if (li < W + 1) Fg[li] = 0u;
+                } else {
+                    for (int w = li; w < W + 1; w += L) Fg[w] = 0u;
+                }
+                float bc, bs, ec, es;
+                float Sv, Sp, Bc, Bs, Ec, Es;   // group totals in diag mode
+                if (KH) {
+                    // Kahan-compensated ray sums over the lane's nibble entries,
+                    // then the group's sums of the compensated values in float64
+                    float ks[4], kc[4];
+                    span_sums_kahan<RPL, L>(tb, ts, v, sp, ks, kc);
+                    const double dBc = group_sum_f64<G>((double)ks[0] - (double)kc[0]);
+                    const double dBs = group_sum_f64<G>((double)ks[1] - (double)kc[1]);
+                    const double dEc = group_sum_f64<G>((double)ks[2] - (double)kc[2]);
+                    const double dEs = group_sum_f64<G>((double)ks[3] - (double)kc[3]);
+                    Sv = (float)((double)fd.y * -dBc + (double)(fd.y * C.alp1) * dEc);
+                    Sp = (float)((double)fd.z * -dBs + (double)(fd.z * C.bet1) * dEs);
+                    Bc = (float)dBc; Bs = (float)dBs; Ec = (float)dEc; Es = (float)dEs;
+                } else {
+                span_sums<RPL, L>(tb, ts, v, sp, &bc, &bs, &ec, &es);
+                Bc = bc; Bs = bs; Ec = ec; Es = es;
+                if (!want_diag) {
+                    Sv = fd.y * (-bc) + (fd.y * C.alp1) * ec;
+                    Sp = fd.z * (-bs) + (fd.z * C.bet1) * es;
+                    group_sum2<G>(Sv, Sp);
+                } else {
+                    group_sum2<G>(Bc, Bs);
+                    group_sum2<G>(Ec, Es);
+                    Sv = fd.y * -Bc + (fd.y * C.alp1) * Ec;
+                    Sp = fd.z * -Bs + (fd.z * C.bet1) * Es;
+                }
+                }
+
+                // ------------------------------------------------ motion
+                float vel, o, dv, dpsi;
+                steer(C, fth, fd.x, fd.w, Sv, Sp, &o, &vel, &dv, &dpsi);
+                const float ang = li == L - 2 ? fpn.w : o;
+                float s_l, c_l;
+                sincos_f32(ang, &s_l, &c_l);
+                float nx, ny;
+                euler_boundary(C, fx, fy, fr, vel, s_l, c_l, &o, &nx, &ny);
+                if (li == L - 1 && kact) {
+                    poseg[k] = make_float4(nx, ny, fr, o);
+                    dyng[k].x = vel;
+                    if (want_diag) {
+                        float* dd = diag + 6 * (int64_t)(a0 + k);
+                        dd[0] = dv; dd[1] = dpsi;
+                        dd[2] = C.phi_ok ? fd.y * -Bc : 0.f;
+                        dd[3] = C.phi_ok ? (fd.y * C.alp1) * Ec : 0.f;
+                        dd[4] = C.phi_ok ? fd.z * -Bs : 0.f;
+                        dd[5] = C.phi_ok ? (fd.z * C.bet1) * Es : 0.f;
+                    }
+                }
+                sth = group_bcast_second_last<G>(s_l, gl);
+                cth = group_bcast_second_last<G>(c_l, gl);
+                fp = fpn;
+                fd = fdn;
+                wave_sync();
             }
-            sth = group_bcast_second_last<G>(s_l, gl);
-            cth = group_bcast_second_last<G>(c_l, gl);
-            fp = fpn;
-            fd = fdn;
-            wave_sync();
         }
         for (int i = li; i < myN; i += L) {
             const int g = a0 + i;
@@ -314,7 +323,7 @@ vf_group_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ grou
     }
 }
 
-template <int G, int RPL, bool KH>
+template <int G, int RPL, bool KH, bool PLAIN>
 int launch_group_shape(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                        uint32_t flags, hipStream_t s) {
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
@@ -331,7 +340,7 @@ int launch_group_shape(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bi
     if (lds > (size_t)b->dev_max_lds)
         return set_error(ABM_ERR_UNSUPPORTED, "abm_vf_step_f32 (grouped): %zu B of LDS needed (device max %d)",
                          lds, b->dev_max_lds);
-    auto kern = vf_group_kernel<G, RPL, KH>;
+    auto kern = vf_group_kernel<G, RPL, KH, PLAIN>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -365,10 +374,13 @@ int launch_vf_group_f32(abm_vf_batch* b, int G, const abm_vf_state_f32* st, uint
                         float* diag, uint32_t flags, hipStream_t s) {
     const int rpl = group_rpl(G, b->max_rays);
     const bool kahan = (flags & ABM_KAHAN_SUMS) != 0;
+    const bool plain = bits == nullptr &&
+                       (flags & (ABM_WANT_DIAG | ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
 #define ABM_GLAUNCH(GV, RV) \
     if (G == GV && rpl == RV) \
-        return kahan ? launch_group_shape<GV, RV, true>(b, st, bits, diag, flags, s) \
-                     : launch_group_shape<GV, RV, false>(b, st, bits, diag, flags, s);
+        return kahan ? launch_group_shape<GV, RV, true, false>(b, st, bits, diag, flags, s) \
+                     : (plain ? launch_group_shape<GV, RV, false, true>(b, st, bits, diag, flags, s) \
+                              : launch_group_shape<GV, RV, false, false>(b, st, bits, diag, flags, s));
     ABM_GLAUNCH(1, 4) ABM_GLAUNCH(1, 8) ABM_GLAUNCH(1, 16)
     ABM_GLAUNCH(2, 4) ABM_GLAUNCH(2, 8) ABM_GLAUNCH(2, 16) ABM_GLAUNCH(2, 32)
     ABM_GLAUNCH(4, 4) ABM_GLAUNCH(4, 8) ABM_GLAUNCH(4, 16) ABM_GLAUNCH(4, 32) ABM_GLAUNCH(4, 64)

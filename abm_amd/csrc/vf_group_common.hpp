@@ -21,7 +21,8 @@ __device__ inline void wave_sync() {
 // SIMD would finish one after another and the last would run alone,
 // latency-bound; lowering the priority of waves that have advanced lets the
 // younger ones keep up, and the short last stage keeps the staggered finish
-// short.  Changes only at stage boundaries (k wave-uniform).
+// short.  The focal loop runs stage by stage, so the priority changes only
+// at stage boundaries.
 struct ProgressStages {
     int k1, k2, k3;   // first focal agent of stages 1, 2, 3
 };
@@ -29,11 +30,15 @@ __device__ inline ProgressStages progress_stages(int n) {
     // stage(k) = 0 for 16 k < 8 n, 1 for 16 k < 12 n, 2 for 16 k < 15 n, else 3
     return {(8 * n + 15) / 16, (12 * n + 15) / 16, (15 * n + 15) / 16};
 }
-__device__ inline void progress_priority(int k, const ProgressStages& ps) {
-    if (k == 0) __builtin_amdgcn_s_setprio(3);
-    else if (k == ps.k3) __builtin_amdgcn_s_setprio(0);
-    else if (k == ps.k2) __builtin_amdgcn_s_setprio(1);
-    else if (k == ps.k1) __builtin_amdgcn_s_setprio(2);
+// Sets the priority of stage s (0..3) and returns the stage's end (the first
+// focal agent of the next stage, or n).
+__device__ inline int stage_priority(int s, const ProgressStages& ps, int n) {
+    switch (s) {
+        case 0: __builtin_amdgcn_s_setprio(3); return ps.k1;
+        case 1: __builtin_amdgcn_s_setprio(2); return ps.k2;
+        case 2: __builtin_amdgcn_s_setprio(1); return ps.k3;
+        default: __builtin_amdgcn_s_setprio(0); return n;
+    }
 }
 
 // Value of lane (group base + L - 2) in every lane of the group.

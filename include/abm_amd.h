@@ -186,7 +186,11 @@ int abm_vf_f64_near_ties(abm_vf_batch* batch, int64_t* near_ties, void* stream);
  * kernel for every replica (32; ABM_ERR_UNSUPPORTED at step time when the
  * fields do not fit in LDS), the chain step for every replica in the
  * reference order (64; allocates its buffers when the batch has none: call it
- * outside graph capture), or the automatic choice (0). */
+ * outside graph capture), or the automatic choice (0).  The 16-wave and push
+ * kernels and the chain step run the reference order only: a step with
+ * ABM_ORDER_SYNCHRONOUS under shapes 16, 32 or 64 returns
+ * ABM_ERR_UNSUPPORTED, and the automatic choice runs it on the per-replica
+ * (or grouped, or for flocks above 2048 agents the wide) kernel. */
 int abm_vf_set_launch(abm_vf_batch* batch, int32_t waves_per_group);
 /* The kernels the batch's last step launched (ABM_KERNEL_* bits), for
  * callers that log or check the launch choice.  No counterpart in the

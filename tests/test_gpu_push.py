@@ -96,3 +96,26 @@ def test_large_class_beyond_lds_takes_the_16_wave_kernel(gpu):
     sw.set_launch(32)
     with pytest.raises(AbmUnsupported, match="push kernel"):
         sw.step()
+
+
+def test_synchronous_order_stays_off_the_reference_order_kernels(gpu):
+    # the push and 16-wave kernels and the chain step move agents in id
+    # order only: a synchronous step of a batch with a large size class runs
+    # on the per-replica kernel, with the float64 kernel's synchronous fields
+    # bit for bit; a forced reference-order shape refuses it
+    import torch
+    from abm_amd import AbmUnsupported
+    from abm_amd.vf import VFSwarm
+    cfgs, states = batch(SPEC[1:], 65)
+    cat = [np.concatenate([s[i] for s in states]) for i in range(4)]
+    a = VFSwarm(cfgs, *cat, radius=5, precision="f32")
+    b = VFSwarm(cfgs, *[x.astype(np.float32).astype(np.float64) for x in cat], radius=5, precision="f64")
+    a.step(order="synchronous", fields=True)
+    b.step(order="synchronous", fields=True)
+    assert a.last_kernels() == {"per_replica"}
+    torch.cuda.synchronize()
+    assert torch.equal(a.fields, b.fields)
+    for shape in (16, 32, 64):
+        a.set_launch(shape)
+        with pytest.raises(AbmUnsupported, match="reference order only"):
+            a.step(order="synchronous")

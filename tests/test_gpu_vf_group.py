@@ -261,3 +261,30 @@ def test_tie_pairs_decided_like_float64(gpu, shape):
     for b in range(4):
         sl = sw2.replica_slice(b)
         per_agent_check(sw2, b, cfgs[b], states[b], tuple(a[sl] for a in after), max_agents=3)
+
+
+@pytest.mark.parametrize("G", [1, 2, 4])
+def test_plain_step_equals_step_with_outputs(gpu, G):
+    # a step that asks for no field, diagnostics or counters runs the kernel's
+    # plain instantiation (the output tests compiled out); its poses and
+    # speeds equal those of the same kernel writing fields, and of a
+    # verify-mode step, bit for bit, step after step (ragged N, ALP0 / BET0
+    # per replica, torus).  (Diagnostics total the four sums separately, a
+    # rounding apart from the two integrals, so they are not compared here.)
+    import torch
+    cfgs, states = sweep_batch([50, 37, 64, 12, 50, 61, 3, 50] * 8, seed=9)
+    plain, full, ver = swarm(cfgs, states), swarm(cfgs, states), swarm(cfgs, states)
+    for s in (plain, full, ver):
+        s.set_grouping(G)
+    for step in range(6):
+        print(f"  plain vs outputs G={G} step {step}", flush=True)
+        plain.step()
+        full.step(fields=True)
+        ver.step(verify=True)
+        assert plain.last_kernels() == {"grouped"} and full.last_kernels() == {"grouped"}
+        torch.cuda.synchronize()
+        for s in (full, ver):
+            for p, q in zip(plain.state_numpy(), s.state_numpy()):
+                assert np.array_equal(p, q), step
+    _, _, disagree = ver.fallback_counts()
+    assert disagree == 0
