@@ -102,7 +102,7 @@ struct abm_vf_batch {
     int device = 0;
     int dev_max_lds = 0;            // device limits, read once at creation
     int n_cu = 0;
-    int occ_f32[2][2][3] = {};      // [size classes][sync][nw 1,2,4]: groups per CU
+    int occ_f32[2][2][2][3] = {};   // [size classes][sync][plain][nw 1,2,4]: groups per CU
     abm::VFRep* reps_dev = nullptr;
     double* tab_cos = nullptr;      // float64 tables: cos(PHI), sin(PHI), diff(PHI)
     double* tab_sin = nullptr;

@@ -63,7 +63,7 @@ __device__ inline Smem carve(void* base, int maxN, int tab_entries, int nw, bool
     return s;
 }
 
-template <int NW, int RPL>
+template <int NW, int RPL, bool PLAIN>   // PLAIN: as the push kernel's
 __global__ void __launch_bounds__(64 * NW)
 vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order, int n_rep,
                    const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
@@ -74,9 +74,9 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool sync_order = (flags & ABM_ORDER_SYNCHRONOUS) != 0;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     Smem S = carve(smem_d, maxN, RPL * NT, NW, sync_order);
     int loaded_tab = -1;
     uint32_t n_pairs = 0, n_recheck = 0, n_disagree = 0;   // this lane's
@@ -223,7 +223,7 @@ vf_step_f32_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ o
                     Bc = comp[0]; Bs = comp[1]; Ec = comp[2]; Es = comp[3];
                 }
             }
-            if (bits_out) {
+            if (!PLAIN && bits_out) {
                 uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
                 for (int w = tid; w < C.W; w += NT) dst[w] = S.F[w];
             }
@@ -330,7 +330,7 @@ struct WideSmem {
 // START: the chain step's first pass instead (see vf_step_f32_chain_kernel):
 // agent k's pairs with the agents after it in id order only, its field written
 // to start_fields[k W ..] and its heading vector to start_u[k]; no move.
-template <int RPL, bool START>
+template <int RPL, bool START, bool PLAIN>   // PLAIN: as the push kernel's
 __global__ void __launch_bounds__(64 * kWideWaves)
 vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
                         abm_vf_state_f32 st, float4* __restrict__ out_pose,
@@ -342,9 +342,9 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     extern __shared__ __align__(16) double smem_d[];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     // Grid path (counters and verify mode keep the full scan): focal agents
     // in cell order, so a block's waves read the same neighbourhoods
     const WideGrid G = *grid;
@@ -567,7 +567,7 @@ vf_step_f32_wide_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict
     const double Bc = wave_total(bc), Bs = wave_total(bs), Ec = wave_total(ec), Es = wave_total(es);
     const double Sv = (double)alp0 * -Bc + (double)(alp0 * C.alp1) * Ec;
     const double Sp = (double)bet0 * -Bs + (double)(bet0 * C.bet1) * Es;
-    if (bits_out) {
+    if (!PLAIN && bits_out) {
         uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
         for (int w = lane; w < C.W; w += 64) dst[w] = F[w];
     }
@@ -724,7 +724,8 @@ template <int RPL>
 int launch_wide(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                 uint32_t flags, hipStream_t s) {
     const size_t lds = lds_bytes_wide(RPL);
-    auto kern = vf_step_f32_wide_kernel<RPL, false>;
+    const bool plain = bits == nullptr && (flags & (ABM_WANT_DIAG | ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
+    auto kern = plain ? vf_step_f32_wide_kernel<RPL, false, true> : vf_step_f32_wide_kernel<RPL, false, false>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -1158,7 +1159,9 @@ __device__ inline void push_culled_check(float4 f, float4 o, const VFRep* rp, bo
     }
 }
 
-template <int RPL>
+// PLAIN: a step with no fields, diagnostics or counters out (the flags and
+// pointers that ask for them are ignored), so their tests leave the kernel.
+template <int RPL, bool PLAIN>
 __global__ void __launch_bounds__(kPushThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 vf_step_f32_push_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order,
                         const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
@@ -1169,9 +1172,9 @@ vf_step_f32_push_kernel(const VFRep* __restrict__ reps, const int32_t* __restric
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int W1 = C.W + 1;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     // laid out for this replica (lds_bytes_push(N, R) bytes)
     char* p = reinterpret_cast<char*>(smem_d);
     float4* P0 = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)C.N;    // start poses
@@ -1257,7 +1260,7 @@ vf_step_f32_push_kernel(const VFRep* __restrict__ reps, const int32_t* __restric
             const float alp0 = dk.y, bet0 = dk.z, v0 = dk.w;
             double Sv, Sp, Bc, Bs, Ec, Es;
             one_wave_sums<RPL>(Fk, T, lane, C, want_diag, alp0, bet0, Sv, Sp, Bc, Bs, Ec, Es);
-            if (bits_out) {
+            if (!PLAIN && bits_out) {
                 uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
                 for (int w = lane; w < C.W; w += 64) dst[w] = Fk[w];
             }
@@ -1353,7 +1356,8 @@ bool push_fits_in(const abm_vf_batch* b, bool all) {
 template <int RPL>
 int launch_push(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag,
                 uint32_t flags, hipStream_t s, const int32_t* order, int n_rep, size_t lds) {
-    auto kern = vf_step_f32_push_kernel<RPL>;
+    const bool plain = bits == nullptr && (flags & (ABM_WANT_DIAG | ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
+    auto kern = plain ? vf_step_f32_push_kernel<RPL, true> : vf_step_f32_push_kernel<RPL, false>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -1392,7 +1396,7 @@ int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
 // id-order chain and over the whole GPU:
 //   1. wide_grid_kernel: the grid of the start poses (torus arenas whose
 //      cells can be at least the visibility cut-off wide).
-//   2. vf_step_f32_wide_kernel<RPL, true>: each agent's field with the agents
+//   2. vf_step_f32_wide_kernel<RPL, true, *>: each agent's field with the agents
 //      after it, all at their start poses, into a global buffer of fields,
 //      and its start heading vector (one wave per agent).
 //   3. For each segment of kChainSeg agents in id order:
@@ -1436,7 +1440,7 @@ __device__ inline void chain_barrier() {
 // behind an extra barrier, before wave 0 uses it.  The decisions are exact
 // and the sums and moves are the push kernel's, so the two give the same
 // fields and poses bit for bit.
-template <int RPL>
+template <int RPL, bool PLAIN>   // PLAIN: as the push kernel's
 __global__ void __launch_bounds__(kChainThreads)
 vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restrict__ tab32,
                            abm_vf_state_f32 st, const uint32_t* __restrict__ fields,
@@ -1453,9 +1457,9 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
     __shared__ uint32_t queue[kChainWaves][kChainQueue];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     const float a0d = (float)rp->alp0, b0d = (float)rp->bet0, v0d = (float)rp->v0;
     for (int i = tid; i < n; i += kChainThreads) {
         const int g = C.a0 + s0 + i;
@@ -1553,7 +1557,7 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
         } else if (wave == 1) {
             if (kk + 1 < n) sums(kk + 1, buf ^ 1);   // lacks only the pair (k + 1, k)
         } else {
-            if (bits_out && wave == 2 && lane < C.W) bits_out[C.field_offset + (int64_t)k * C.W + lane] = rows[kk][lane];
+            if (!PLAIN && bits_out && wave == 2 && lane < C.W) bits_out[C.field_offset + (int64_t)k * C.W + lane] = rows[kk][lane];
             if (kk >= 1) {
                 // agent k - 1's new pose into the fields of the segment's
                 // agents from k + 2 on: the cut-off test 64 at a time, the rest
@@ -1694,7 +1698,8 @@ template <int RPLW>
 int launch_chain_start(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t flags, hipStream_t s,
                        int rep, float cull_c, int n_agents) {
     const size_t lds = lds_bytes_wide(RPLW);
-    auto kern = vf_step_f32_wide_kernel<RPLW, true>;
+    const bool plain = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
+    auto kern = plain ? vf_step_f32_wide_kernel<RPLW, true, true> : vf_step_f32_wide_kernel<RPLW, true, false>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -1721,6 +1726,7 @@ bool chain_fits(const abm_vf_batch* b) { return chain_ready(b) && b->min_agents 
 int launch_chain(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float* diag, uint32_t flags,
                  hipStream_t s) {
     const bool counting = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0;
+    const bool plain = bits == nullptr && !counting && (flags & ABM_WANT_DIAG) == 0;
     const int rpl = push_rpl(b->max_rays);
     for (int rep = 0; rep < b->n_replicas; ++rep) {
         const VFRep& h = b->reps_host[rep];
@@ -1741,7 +1747,9 @@ int launch_chain(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, fl
             // rays per lane of the summing wave: the push kernel's rule for the batch
 #define ABM_SEGMENT(RPLV)                                                                                   \
     if (rpl == RPLV)                                                                                        \
-        hipLaunchKernelGGL(vf_step_f32_segment_kernel<RPLV>, dim3(1), dim3(kChainThreads), 0, s,           \
+        hipLaunchKernelGGL((plain ? vf_step_f32_segment_kernel<RPLV, true>                                  \
+                                  : vf_step_f32_segment_kernel<RPLV, false>),                               \
+                           dim3(1), dim3(kChainThreads), 0, s,                                              \
                            b->reps_dev + rep, b->tab32, *st, (const uint32_t*)b->chain_fields,             \
                            (const float2*)b->chain_u, bits, diag, flags, b->counters, cull_c, s0, n);
             ABM_SEGMENT(4) ABM_SEGMENT(8) ABM_SEGMENT(16) ABM_SEGMENT(32)
@@ -1787,9 +1795,10 @@ int launch_nw(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, float
         return set_error(ABM_ERR_UNSUPPORTED,
                          "abm_vf_step_f32: %d agents x %d rays need %zu B of LDS (device max %d)",
                          maxN, maxR, lds, b->dev_max_lds);
-    auto kern = vf_step_f32_kernel<NW, RPL>;
+    const bool plain = bits == nullptr && (flags & (ABM_WANT_DIAG | ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
+    auto kern = plain ? vf_step_f32_kernel<NW, RPL, true> : vf_step_f32_kernel<NW, RPL, false>;
     const int slot = NW == 1 ? 0 : (NW == 2 ? 1 : 2);
-    int& per_cu = b->occ_f32[split ? 1 : 0][sync_order ? 1 : 0][slot];
+    int& per_cu = b->occ_f32[split ? 1 : 0][sync_order ? 1 : 0][plain ? 1 : 0][slot];
     if (per_cu == 0) {   // first launch of this shape: set the LDS limit, read the occupancy
         if (lds > 65536) {
             if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
