@@ -762,7 +762,10 @@ __device__ inline double words_to_double(uint32_t w0, uint32_t w1) {
 // The random walk's uniforms come either as doubles (uniforms) or as raw
 // 32-bit outputs (words, two per draw, starting after the words the resource
 // loop consumed: u_used[b] on entry).  With status, only replicas marked
-// kWorldReady are updated, and they are marked ABM_WORLD_DONE.
+// kWorldReady are updated, and they are marked ABM_WORLD_DONE.  FIELDS false:
+// a step that writes no fields (bits_out is ignored), so the field paths'
+// tests leave the agent loop.
+template <bool FIELDS>
 __global__ void __launch_bounds__(512)
 forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
@@ -772,6 +775,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    abm_mt19937_state* __restrict__ mt, int maxN, int maxR) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) double smem[];
+    if (!FIELDS) bits_out = nullptr;
     // blocks take the replicas largest first, so the longest id-order chains
     // start at once and the small replicas fill in behind them
     const int rb = order[blockIdx.x];
@@ -2466,13 +2470,14 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     if (b->n_small > 0) {
         const int nthr = b->small_max_agents > 128 ? 512 : 256;
         const size_t lds = fg_lds_bytes(b->small_max_agents, b->small_max_rays);
+        auto kern = field_bits ? forage_step_kernel<true> : forage_step_kernel<false>;
         if (lds > 65536) {
-            if (int rc = check_hip(hipFuncSetAttribute((const void*)forage_step_kernel,
+            if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                                    "hipFuncSetAttribute"))
                 return rc;
         }
-        hipLaunchKernelGGL(forage_step_kernel, dim3(b->n_small), dim3(nthr), lds, (hipStream_t)stream,
+        hipLaunchKernelGGL(kern, dim3(b->n_small), dim3(nthr), lds, (hipStream_t)stream,
                            (const FGRep*)b->reps_dev, *st, uniforms, words, offset, count, used, status,
                            field_bits, (const int32_t*)b->order_dev, mt, b->small_max_agents,
                            b->small_max_rays);
