@@ -1412,7 +1412,8 @@ int launch_push_any(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits,
 // Kernel boundaries order the global fields between the launches.
 constexpr int kChainThreads = 512;
 constexpr int kChainWaves = kChainThreads / 64;
-constexpr int kChainHelpers = kChainWaves - 1;
+constexpr int kChainCullWaves = 3;    // segment kernel: three waves test, three decide
+static_assert(kChainWaves == 2 + 2 * kChainCullWaves, "segment kernel wave roles");
 constexpr int kChainSeg = 256;        // agents per segment
 constexpr int kChainQueue = 128;      // queued agents per helper wave
 constexpr int kChainPushWaves = 4;    // waves per workgroup of the push kernel
@@ -1430,12 +1431,18 @@ __device__ inline void chain_barrier() {
 // agent k:
 //   * wave 0 takes agent k's sums (made by wave 1 during agent k - 1), moves
 //     agent k and decides the pairs (k + 1, k) and (k + 2, k) at k's new pose
-//     (one lane each);
+//     and (k + 2, k - 1) at k - 1's (one lane each);
 //   * wave 1 sums agent k + 1's field, which then lacks only the pair
 //     (k + 1, k), as the push kernel's wave 0 sums (one_wave_sums);
-//   * waves 2..7 decide agent k - 1's new pose in the fields of the
-//     segment's agents from k + 2 on (the cheap float32 cut-off test, the rest
-//     queued and decided 64 at a time).
+//   * three waves test agent k - 1's new pose against the segment's agents
+//     from k + 3 on (the cheap float32 cut-off test) and list the pairs that
+//     remain; three more decide the pairs listed in the previous step (agent
+//     k - 2's new pose), 64 per wave at a time.  A pair (m, j) with
+//     m >= j + 4 is so decided two steps after j moved, before wave 1 sums
+//     agent m's field (in step m - 1), and each wave makes one pass a step.
+//     The busiest of them (the first lister and the first decider) run as
+//     waves 3 and 2, which share their SIMDs (wave w on SIMD w mod 4) with
+//     the lightly loaded waves 7 and 6, not with waves 0 and 1.
 // When the pair (k + 1, k) draws, wave 1 sums agent k + 1's field again,
 // behind an extra barrier, before wave 0 uses it.  The decisions are exact
 // and the sums and moves are the push kernel's, so the two give the same
@@ -1452,9 +1459,10 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
     __shared__ float2 su[kChainSeg];       // start heading vectors
     __shared__ float4 sdyn[kChainSeg];     // speed, ALP0, BET0, V0
     __shared__ double red[2][6];           // Sv, Sp, Bc, Bs, Ec, Es of agents k and k + 1
-    __shared__ float4 newpose[2];
+    __shared__ float4 newpose[4];          // new pose of local agent kk at kk & 3
     __shared__ int dirty[2];               // the pair (k, k - 1) drew after k's sums were taken
-    __shared__ uint32_t queue[kChainWaves][kChainQueue];
+    __shared__ uint16_t cq[2][kChainCullWaves][kChainSeg];   // listed rows, by step parity and cull wave
+    __shared__ int cn[2][kChainCullWaves];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
@@ -1494,7 +1502,6 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
         }
     };
     unsigned long long n_pairs = 0, n_recheck = 0, n_disagree = 0;
-    uint32_t* wq = queue[wave];
     __syncthreads();
     if (wave == 1) sums(0, 0);   // agent s0's field is complete
     __syncthreads();
@@ -1523,7 +1530,7 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
             euler_boundary(C, fx, fy, fr, vel, so, co, &o, &nx, &ny);
             const float4 moved = make_float4(nx, ny, fr, o);
             if (lane == 0) {
-                newpose[kk & 1] = moved;
+                newpose[kk & 3] = moved;
                 st.x[gk] = nx;
                 st.y[gk] = ny;
                 st.theta[gk] = o;
@@ -1537,16 +1544,18 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
                     dd[5] = C.phi_ok ? (float)((double)(bet0 * C.bet1) * Es) : 0.f;
                 }
             }
-            // lane d (1, 2): the pair (k + d, k)
+            // lane d (1, 2): the pair (k + d, k); lane 3: the pair (k + 2, k - 1)
+            // at k - 1's new pose (agent s0 - 1's pairs came with the fields)
             bool drew = false;
-            if (lane >= 1 && lane <= 2 && kk + lane < n) {
-                const int m = kk + lane;
+            const int m = kk + (lane == 3 ? 2 : lane);
+            if (lane >= 1 && lane <= 3 && m < n && (lane < 3 || kk >= 1)) {
                 const float4 f = srec[m];
-                if (push_culled(f, moved, C, cull_c)) {
-                    push_culled_check(f, moved, rp, count, verify, n_pairs, n_disagree);
+                const float4 ob = lane == 3 ? newpose[(kk - 1) & 3] : moved;
+                if (push_culled(f, ob, C, cull_c)) {
+                    push_culled_check(f, ob, rp, count, verify, n_pairs, n_disagree);
                 } else {
                     PairF32 pr;
-                    if (decide_pair(pr, su[m], f, moved, rp, C, count, verify, n_pairs, n_recheck, n_disagree)) {
+                    if (decide_pair(pr, su[m], f, ob, rp, C, count, verify, n_pairs, n_recheck, n_disagree)) {
                         raster(rows[m], pr.start, pr.end, C.R);
                         drew = true;
                     }
@@ -1556,15 +1565,15 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
             if (lane == 0) dirty[buf ^ 1] = dm != 0 ? 1 : 0;
         } else if (wave == 1) {
             if (kk + 1 < n) sums(kk + 1, buf ^ 1);   // lacks only the pair (k + 1, k)
-        } else {
-            if (!PLAIN && bits_out && wave == 2 && lane < C.W) bits_out[C.field_offset + (int64_t)k * C.W + lane] = rows[kk][lane];
+        } else if (wave >= 3 && wave <= 5) {
+            if (!PLAIN && bits_out && wave == 3 && lane < C.W) bits_out[C.field_offset + (int64_t)k * C.W + lane] = rows[kk][lane];
+            // agent k - 1's new pose against the segment's agents from k + 3
+            // on: the cut-off test, the rest listed for the next step
+            const int c = wave - 3;
+            int cnt = 0;   // wave-uniform
             if (kk >= 1) {
-                // agent k - 1's new pose into the fields of the segment's
-                // agents from k + 2 on: the cut-off test 64 at a time, the rest
-                // queued in a ring of 128 and decided 64 at a time
-                const float4 o = newpose[(kk - 1) & 1];
-                int head = 0, tail = 0;   // wave-uniform ring positions
-                for (int mb = kk + 2 + (wave - 2) * 64; mb < n; mb += (kChainWaves - 2) * 64) {
+                const float4 o = newpose[(kk - 1) & 3];
+                for (int mb = kk + 3 + c * 64; mb < n; mb += kChainCullWaves * 64) {
                     const int m = mb + lane;
                     bool keep = false;
                     if (m < n) {
@@ -1575,22 +1584,20 @@ vf_step_f32_segment_kernel(const VFRep* __restrict__ rp, const RayTab32* __restr
                         }
                     }
                     const uint64_t mk = __ballot(keep);
-                    if (keep)
-                        wq[(tail + (int)__popcll(mk & ((1ull << lane) - 1ull))) & (kChainQueue - 1)] = (uint32_t)m;
-                    tail += (int)__popcll(mk);
-                    wave_lds_sync();
-                    if (tail - head >= 64) {
-                        const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
-                        push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck,
-                                  n_disagree);
-                        head += 64;
-                        wave_lds_sync();
-                    }
+                    if (keep) cq[buf][c][cnt + (int)__popcll(mk & ((1ull << lane) - 1ull))] = (uint16_t)m;
+                    cnt += (int)__popcll(mk);
                 }
-                if (lane < tail - head) {
-                    const int mm = (int)wq[(head + lane) & (kChainQueue - 1)];
-                    push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
-                }
+            }
+            if (lane == 0) cn[buf][c] = cnt;
+        } else if (kk >= 2) {
+            // the pairs listed in the previous step, at agent k - 2's new pose
+            const int pb = buf ^ 1;
+            const int c0 = cn[pb][0], c1 = cn[pb][1], c2 = cn[pb][2];
+            const float4 o = newpose[(kk - 2) & 3];
+            const int d = wave == 2 ? 0 : wave - 5;   // waves 2, 6, 7
+            for (int e = d * 64 + lane; e < c0 + c1 + c2; e += kChainCullWaves * 64) {
+                const int mm = e < c0 ? cq[pb][0][e] : (e < c0 + c1 ? cq[pb][1][e - c0] : cq[pb][2][e - c0 - c1]);
+                push_pair(rows[mm], su[mm], srec[mm], o, rp, C, count, verify, n_pairs, n_recheck, n_disagree);
             }
         }
         chain_barrier();
