@@ -763,9 +763,10 @@ __device__ inline double words_to_double(uint32_t w0, uint32_t w1) {
 // 32-bit outputs (words, two per draw, starting after the words the resource
 // loop consumed: u_used[b] on entry).  With status, only replicas marked
 // kWorldReady are updated, and they are marked ABM_WORLD_DONE.  FIELDS false:
-// a step that writes no fields (bits_out is ignored), so the field paths'
-// tests leave the agent loop.
-template <bool FIELDS>
+// a step that writes no fields (bits_out is ignored); EXCL false: no replica
+// of the launch has visual exclusion.  Either takes the other paths' tests
+// out of the agent loop.
+template <bool FIELDS, bool EXCL>
 __global__ void __launch_bounds__(512)
 forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
                    const double* __restrict__ uniforms, const uint32_t* __restrict__ words,
@@ -848,7 +849,7 @@ forage_step_kernel(const FGRep* __restrict__ reps, abm_forage_state st,
     int draws = 0;   // meaningful in thread 0
     int xlive = 0, xs = 0, xe = 0;   // thread 0: agent k's social interval for agent k+1
     const int nl = R / 2;
-    const bool excl = P.visual_exclusion != 0;
+    const bool excl = EXCL && P.visual_exclusion != 0;
     const FastRec FR = fast_rec_const(R, P.step, P.fov0, P.fov1);
     // without exclusion and field output, wave 0 rasters and counts alone
     const bool wave_raster = !excl && !bits_out;
@@ -2470,7 +2471,9 @@ int launch_forage_step(abm_forage_batch* b, const abm_forage_state* st, const do
     if (b->n_small > 0) {
         const int nthr = b->small_max_agents > 128 ? 512 : 256;
         const size_t lds = fg_lds_bytes(b->small_max_agents, b->small_max_rays);
-        auto kern = field_bits ? forage_step_kernel<true> : forage_step_kernel<false>;
+        const bool ex = std::any_of(b->exclusion.begin(), b->exclusion.end(), [](uint8_t e) { return e != 0; });
+        auto kern = field_bits ? (ex ? forage_step_kernel<true, true> : forage_step_kernel<true, false>)
+                               : (ex ? forage_step_kernel<false, true> : forage_step_kernel<false, false>);
         if (lds > 65536) {
             if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
