@@ -119,3 +119,20 @@ def test_synchronous_order_stays_off_the_reference_order_kernels(gpu):
         a.set_launch(shape)
         with pytest.raises(AbmUnsupported, match="reference order only"):
             a.step(order="synchronous")
+
+
+def test_size_class_rule_follows_the_cus(gpu):
+    # the automatic launch sends the large class (replicas above 64 agents)
+    # to the push kernel, one workgroup per CU, only while the class's agents
+    # spread over the CUs do not outlast its largest replica's chain; a class
+    # of many similar replicas stays on the per-replica kernel
+    import torch
+    from abm_amd.vf import VFConfig, VFSwarm
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    for n_rep, kernels in ((max(1, n_cu // 2), {"push"}), (2 * n_cu, {"per_replica"})):
+        cfg = VFConfig(n_agents=80, n_rays=256, width=900.0, height=900.0, boundary="infinite",
+                       ALP0=1.0, BET0=0.5)
+        st = uniform_start(80 * n_rep, 900.0, 900.0, 30, seed=68, v=0.5)
+        sw = VFSwarm([cfg] * n_rep, *st, radius=5, precision="f32")
+        sw.step()
+        assert sw.last_kernels() == kernels, (n_rep, sw.last_kernels())
