@@ -1415,7 +1415,7 @@ constexpr int kChainWaves = kChainThreads / 64;
 constexpr int kChainCullWaves = 3;    // segment kernel: three waves test, three decide
 static_assert(kChainWaves == 2 + 2 * kChainCullWaves, "segment kernel wave roles");
 constexpr int kChainSeg = 256;        // agents per segment
-constexpr int kChainQueue = 128;      // queued agents per helper wave
+constexpr int kChainQueue = 128;      // queued agents per wave of the chain push kernel
 constexpr int kChainPushWaves = 4;    // waves per workgroup of the push kernel
 constexpr int kChainScanRange = 512;  // later agents per wave of the push kernel without a grid
 
