@@ -1835,20 +1835,23 @@ bool push_fits(const abm_vf_batch* b, bool all) { return push_fits_in(b, all); }
 // Replicas per wave of the grouped kernel for this batch, or 0 for the
 // per-replica kernel.  Automatic choice (measured on MI355X): batches with at
 // least 8 replicas per CU run grouped, one replica per wave (four waves per
-// SIMD), or four per wave when that still gives every SIMD two waves; smaller
-// batches are latency-bound and take the per-replica kernel with several
-// waves per replica.
+// SIMD), or four per wave when that still gives every SIMD two waves; so do
+// batches with 4 to 8 replicas per CU of at most 128 agents (one replica per
+// wave: 1024 replicas of 50 agents at R = 512 take 0.070 ms grouped against
+// 0.095 ms on the per-replica kernel, of 100 agents at R = 1024 0.201 against
+// 0.213; of 200 agents the per-replica kernel wins, 0.546 against 0.597);
+// smaller batches are latency-bound and take the per-replica kernel with
+// several waves per replica.
 int selected_group_size(const abm_vf_batch* b) {
     if (!b->group_ok || b->group_g < 0) return 0;
     if (b->group_g > 0) return b->group_g;
     if (b->waves_per_group > 0) return 0;
-    if (b->n_replicas < 8 * b->n_cu) return 0;
+    if (b->n_replicas < 4 * b->n_cu) return 0;
+    if (b->n_replicas < 8 * b->n_cu) return b->max_agents <= 128 ? 1 : 0;
     if (b->gplan[2].n_groups >= 8 * b->n_cu) return 4;
     return 1;
 }
 
-// Waves per replica of the per-replica kernel when not set: as many (up to 4)
-// as leave room for all its replicas at once (16 waves per CU).
 // Waves per replica of the per-replica kernel: four while the batch gives each
 // CU at most two such workgroups, two while it fits at 16 waves per CU, else
 // one (measured on MI355X: 1024 replicas of 100 agents take 0.223 ms on two
