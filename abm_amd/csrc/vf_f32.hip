@@ -785,7 +785,7 @@ size_t lds_bytes_big(int maxN, int maxR) {
     return (b + 15) & ~(size_t)15;
 }
 
-template <int RPL>
+template <int RPL, bool PLAIN>   // PLAIN: as the push kernel's
 __global__ void __launch_bounds__(kBigThreads)
 vf_step_f32_big_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict__ order,
                        const RayTab32* __restrict__ tab32, abm_vf_state_f32 st,
@@ -795,9 +795,9 @@ vf_step_f32_big_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict
     const VFRep* rp = reps + order[blockIdx.x];
     const RepConst C = rep_const(rp);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool want_diag = (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
-    const bool count = (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
-    const bool verify = (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
+    const bool want_diag = !PLAIN && (flags & ABM_WANT_DIAG) != 0 && diag != nullptr;
+    const bool count = !PLAIN && (flags & (ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) != 0 && counters;
+    const bool verify = !PLAIN && (flags & ABM_VERIFY_FAST_PATH) != 0 && counters != nullptr;
     char* p = reinterpret_cast<char*>(smem_d);
     double* red = reinterpret_cast<double*>(p); p += sizeof(double) * (4 * kBigWaves + 8);
     float4* pose = reinterpret_cast<float4*>(p); p += sizeof(float4) * (size_t)maxN;
@@ -968,7 +968,7 @@ vf_step_f32_big_kernel(const VFRep* __restrict__ reps, const int32_t* __restrict
                 }
             }
         }
-        if (bits_out) {
+        if (!PLAIN && bits_out) {
             uint32_t* dst = bits_out + C.field_offset + (int64_t)k * C.W;
             for (int w = tid; w < C.W; w += kBigThreads) dst[w] = F[w];
         }
@@ -1000,7 +1000,8 @@ int launch_big(abm_vf_batch* b, const abm_vf_state_f32* st, uint32_t* bits, floa
                          "(sequential) order on the 16-wave kernel (device max %d); the chain step (the default "
                          "for single flocks, abm_vf_set_launch(64)) takes it", maxN,
                          lds, b->dev_max_lds);
-    auto kern = vf_step_f32_big_kernel<RPL>;
+    const bool plain = bits == nullptr && (flags & (ABM_WANT_DIAG | ABM_COUNT_RECHECKS | ABM_VERIFY_FAST_PATH)) == 0;
+    auto kern = plain ? vf_step_f32_big_kernel<RPL, true> : vf_step_f32_big_kernel<RPL, false>;
     if (lds > 65536) {
         if (int rc = check_hip(hipFuncSetAttribute((const void*)kern,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),

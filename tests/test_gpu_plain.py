@@ -1,7 +1,7 @@
 """Steps that ask for no outputs (no fields, diagnostics or counters) run
 each kernel's plain instantiation, whose output paths are compiled out.  They
 must move every agent exactly as the same kernel does when it writes fields:
-the per-replica kernel, the push kernel and the chain step on a ragged batch
+the per-replica, push and 16-wave kernels and the chain step on a ragged batch
 (reference order, vf_sims.py:302), the per-replica kernel in synchronous
 order, and for a flock above 2048 agents the chain step and the wide kernel
 (synchronous order).  (The grouped kernel's plain step is checked in
@@ -38,6 +38,7 @@ def run(a, b, order, steps, kernels):
     (4, "sequential", {"per_replica"}),
     (1, "synchronous", {"per_replica"}),
     (32, "sequential", {"push"}),
+    (16, "sequential", {"big"}),
     (64, "sequential", {"chain"}),
     (0, "sequential", {"per_replica", "push"}),
 ])
